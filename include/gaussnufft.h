@@ -1,0 +1,111 @@
+/*
+ * gaussnufft.h -- C ABI of the MI355X (gfx950) Gaussian-gridding NUFFT.
+ *
+ * Plain C: opaque plan handle, device pointers, sizes, int status codes.
+ * Complex data are interleaved complex64 (float re, float im).  All device
+ * pointers must be HIP device memory of the process's HIP runtime; the plan
+ * runs every kernel and FFT on the HIP stream it is bound to.
+ *
+ * Reference interfaces replaced (paths in the mripy repository):
+ *   gn_grid_params   <- _compute_2d_grid_params / _compute_3d_grid_params
+ *                       fft/nufft_func.py:215-226, 234-249
+ *   gn_plan_create   <- NUFFT3d_cuda.__init__ pics/operators_cuda_class.py:65-83 and the
+ *                       per-call parameter set-up of nufft3d*_gaussker_cuda
+ *                       fft/nufft_func_cuda.py:1463-1470, 1512-1518
+ *   gn_set_points    <- NUFFT3d_cuda.set_ktraj pics/operators_cuda_class.py:92-96 (x,y,z,dcf)
+ *   gn_execute       <- GN_TYPE2  : nufft3d2_gaussker_cuda fft/nufft_func_cuda.py:1512-1559,
+ *                                   nufft2d2_gaussker_cuda :609-636, NUFFT3d_cuda.forward
+ *                                   pics/operators_cuda_class.py:107-111
+ *                       GN_TYPE1  : density_weighting + nufft3d1_gaussker_cuda
+ *                                   fft/nufft_func_cuda.py:1463-1509, nufft2d1_gaussker_cuda
+ *                                   :581-606, NUFFT3d_cuda.backward operators_cuda_class.py:113-120
+ *                       GN_TYPE21 : nufft3d21_gaussker_cuda fft/nufft_func_cuda.py:1562-1637,
+ *                                   nufft2d21_gaussker_cuda :639-679,
+ *                                   NUFFT3d_cuda.forward_backward operators_cuda_class.py:122-127
+ *                       GN_ADJOINT: exact adjoint of GN_TYPE2 (no counterpart; M * type 1
+ *                                   without density weighting, for autograd and CG)
+ *   gn_plan_destroy  <- (Python garbage collection of the operator object)
+ *   gn_last_error    <- the Python exceptions the reference raises (ValueError for eps,
+ *                       fft/nufft_func.py:200-202)
+ */
+#ifndef GAUSSNUFFT_H
+#define GAUSSNUFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GN_VERSION 1
+
+/* status codes */
+#define GN_OK 0
+#define GN_EINVAL 1  /* invalid argument (sizes, eps range, null pointer, kind) */
+#define GN_ENOMEM 2  /* device allocation failed */
+#define GN_EHIP 3    /* HIP runtime error */
+#define GN_EROCFFT 4 /* rocFFT error */
+#define GN_ESTATE 5  /* call out of order (e.g. execute before set_points) */
+
+/* data layouts of images and samples at the boundary */
+#define GN_COIL_LAST 0  /* reference layout: image (m1,m2[,m3],nc), samples (M,nc), C order */
+#define GN_COIL_MAJOR 1 /* image (nc,m1,m2[,m3]), samples (nc,M) */
+
+/* transform kinds for gn_execute */
+#define GN_TYPE1 1   /* image = (1/M) (pi/tau)^(d/2) e^{tau|k|^2} crop(ifft(spread(dcf*y))) */
+#define GN_TYPE2 2   /* y = (1/G) gather(fft(pad((pi/tau)^(d/2) e^{tau|k|^2} image)))       */
+#define GN_ADJOINT 3 /* exact adjoint of GN_TYPE2: M * GN_TYPE1 without dcf                  */
+#define GN_TYPE21 21 /* GN_TYPE1(GN_TYPE2(image)) with dcf                                   */
+
+typedef struct gn_plan_s gn_plan;
+
+/* Library version (GN_VERSION). */
+int gn_version(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char *gn_last_error(void);
+
+/* Grid parameters from eps, the reference rule (fft/nufft_func.py:198-249):
+ * rat = 2 if eps > 1e-11 else 3; nspread = int(-ln(eps)/(pi(rat-1)/(rat-0.5)) + 0.5),
+ * clamped to [3, max_nspread] when max_nspread > 0 (the 3-D drivers pass 6);
+ * nf[a] = max(rat*modes[a], 2*nspread); tau = pi*nspread/(rat(rat-0.5))/modes[0]^2.
+ * Returns GN_EINVAL unless 1e-33 < eps < 1e-1. */
+int gn_grid_params(int dim, const int64_t *modes, double eps, int max_nspread, int *rat, int *nspread,
+                   int64_t *nf, double *tau);
+
+/* Create a plan for a dim-D (2 or 3) transform of an image of size modes[0..dim-1]
+ * with ncoils coils, oversampling rat and half-width nspread (the window spans
+ * 2*nspread taps per axis).  iflag >= 0: type 2 uses e^{-ik.x}; iflag < 0 flips both
+ * signs.  stream: hipStream_t (NULL = default stream). */
+int gn_plan_create(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,
+                   void *stream, gn_plan **plan);
+
+/* Rebind the plan to another HIP stream. */
+int gn_plan_set_stream(gn_plan *plan, void *stream);
+
+/* Set the M sample coordinates in radians (device pointers; z ignored for dim 2),
+ * float32 when coord_f64 == 0 else float64, each multiplied by df.  dcf: optional
+ * float32 density weights (device pointer, M entries) applied by GN_TYPE1/GN_TYPE21.
+ * Bins and sorts the samples; the caller may free its buffers afterwards. */
+int gn_set_points(gn_plan *plan, int64_t M, const void *x, const void *y, const void *z, int coord_f64, double df,
+                  const float *dcf);
+
+/* Run one transform (kind = GN_TYPE1/GN_TYPE2/GN_ADJOINT/GN_TYPE21) on the plan's
+ * stream.  in/out: device complex64 buffers in the plan's layout.  Asynchronous. */
+int gn_execute(gn_plan *plan, int kind, const void *in, void *out);
+
+/* Query: nf[3] oversampled grid, nspread, tau, number of samples, coils, tile shape. */
+int gn_plan_info(const gn_plan *plan, int64_t *nf, int *nspread, double *tau, int64_t *M, int64_t *ncoils,
+                 int *tile);
+
+/* Override the spread/gather tile shape (cells per axis; 0 keeps the default).
+ * Takes effect at the next gn_set_points. */
+int gn_plan_set_tiles(gn_plan *plan, const int *tile);
+
+int gn_plan_destroy(gn_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAUSSNUFFT_H */

@@ -1,0 +1,71 @@
+"""ctypes binding of libgaussnufft.so (the C ABI in include/gaussnufft.h).
+
+torch is imported before the library is opened so that the HIP runtime torch
+bundles is the one the library binds to.  There is no fallback: if the library
+is missing the import fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads torch's HIP runtime before ours)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgaussnufft.so")
+
+GN_OK, GN_EINVAL, GN_ENOMEM, GN_EHIP, GN_EROCFFT, GN_ESTATE = 0, 1, 2, 3, 4, 5
+GN_COIL_LAST, GN_COIL_MAJOR = 0, 1
+GN_TYPE1, GN_TYPE2, GN_ADJOINT, GN_TYPE21 = 1, 2, 3, 21
+
+# every symbol declared in include/gaussnufft.h, with its ctypes signature
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+SIGNATURES = {
+    "gn_version": ([], ctypes.c_int),
+    "gn_last_error": ([], ctypes.c_char_p),
+    "gn_grid_params": ([ctypes.c_int, _i64p, ctypes.c_double, ctypes.c_int, _ip, _ip, _i64p,
+                        ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "gn_plan_create": ([ctypes.c_int, _i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                        ctypes.c_int, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
+    "gn_plan_set_stream": ([_vp, _vp], ctypes.c_int),
+    "gn_set_points": ([_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
+    "gn_execute": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "gn_plan_info": ([_vp, _i64p, _ip, ctypes.POINTER(ctypes.c_double), _i64p, _i64p, _ip], ctypes.c_int),
+    "gn_plan_set_tiles": ([_vp, _ip], ctypes.c_int),
+    "gn_plan_destroy": ([_vp], ctypes.c_int),
+}
+
+_lib = None
+
+
+class GNError(RuntimeError):
+    pass
+
+
+def load():
+    """Open the library (once) and attach the signatures."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("mripy_amd: %s is not built; run `python -m mripy_amd._build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    """Raise the Python exception matching a GN_* status."""
+    if rc == GN_OK:
+        return
+    msg = load().gn_last_error().decode(errors="replace")
+    if rc == GN_EINVAL:
+        raise ValueError(msg)
+    if rc == GN_ENOMEM:
+        raise MemoryError(msg)
+    raise GNError("gaussnufft status %d: %s" % (rc, msg))

@@ -1,0 +1,392 @@
+// C ABI of the Gaussian-gridding NUFFT (see include/gaussnufft.h).
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gn_internal.h"
+
+namespace gn {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return GN_EHIP;
+}
+
+static int fft_fail(rocfft_status s, const char *what)
+{
+    g_err = std::string("rocFFT ") + what + " failed with status " + std::to_string((int)s);
+    return GN_EROCFFT;
+}
+
+static std::once_flag g_rocfft_once;
+
+// Reference parameter rule, fft/nufft_func.py:198-249.
+static int params_from_eps(int dim, const int64_t *modes, double eps, int max_nspread, int *rat, int *ns)
+{
+    if (!(eps > 1e-33 && eps < 1e-1)) {
+        char buf[128];
+        snprintf(buf, sizeof(buf), "eps = %.0e; must satisfy 1e-33 < eps < 1e-1.", eps);
+        return fail(GN_EINVAL, buf);
+    }
+    const double pi = 3.141592653589793;
+    int r = eps > 1e-11 ? 2 : 3;
+    int n = (int)(-std::log(eps) / (pi * (r - 1) / (r - 0.5)) + 0.5);
+    if (max_nspread > 0) {
+        n = std::min(n, max_nspread);
+        n = std::max(n, 3);
+    }
+    *rat = r;
+    *ns = n;
+    (void)dim;
+    (void)modes;
+    return GN_OK;
+}
+
+static double tau_of(int rat, int ns, int64_t ms)
+{
+    const double pi = 3.141592653589793;
+    const double lam = ns / (rat * (rat - 0.5));
+    return pi * lam / ((double)ms * (double)ms);
+}
+
+static int setup_tiles(gn_plan_s *p)
+{
+    Geometry &g = p->geo;
+    int tile[3] = {1, 1, 1};
+    default_tiles(g.dim, g.ns, g.nf, tile);
+    for (int a = 0; a < g.dim; ++a)
+        if (p->tile_override[a] > 0) tile[a] = (int)std::min<int64_t>(p->tile_override[a], g.nf[a]);
+    g.K = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (a >= g.dim) {
+            g.tile[a] = 1;
+            g.ntiles[a] = 1;
+            continue;
+        }
+        g.tile[a] = tile[a];
+        g.ntiles[a] = (int)((g.nf[a] + tile[a] - 1) / tile[a]);
+        g.K *= g.ntiles[a];
+    }
+    // LDS limits: spread keeps kSpreadWaves tiles of float4, gather stages tile + halo
+    int64_t tc = (int64_t)g.tile[0] * g.tile[1] * g.tile[2];
+    int64_t rc = 1;
+    for (int a = 0; a < g.dim; ++a) rc *= std::min<int64_t>(g.tile[a] + 2 * g.ns - 1, g.nf[a]);
+    if (tc * 16 * kSpreadWaves > 160 * 1024 || rc * 8 > 160 * 1024)
+        return fail(GN_EINVAL, "tile shape exceeds the 160 KiB LDS budget");
+    return GN_OK;
+}
+
+static void destroy_fft(gn_plan_s *p)
+{
+    if (p->fft_fwd) rocfft_plan_destroy(p->fft_fwd);
+    if (p->fft_bwd) rocfft_plan_destroy(p->fft_bwd);
+    if (p->fft_info) rocfft_execution_info_destroy(p->fft_info);
+    if (p->fft_work) hipFree(p->fft_work);
+    p->fft_fwd = p->fft_bwd = nullptr;
+    p->fft_info = nullptr;
+    p->fft_work = nullptr;
+}
+
+static int create_fft(gn_plan_s *p)
+{
+    rocfft_status s;
+    std::call_once(g_rocfft_once, []() { rocfft_setup(); });
+    const Geometry &g = p->geo;
+    size_t lengths[3];
+    for (int a = 0; a < g.dim; ++a) lengths[a] = (size_t)g.nf[g.dim - 1 - a]; // fastest first
+    rocfft_plan_description desc = nullptr;
+    if ((s = rocfft_plan_description_create(&desc)) != rocfft_status_success) return fft_fail(s, "description");
+    size_t strides[3];
+    strides[0] = 1;
+    for (int a = 1; a < g.dim; ++a) strides[a] = strides[a - 1] * lengths[a - 1];
+    s = rocfft_plan_description_set_data_layout(desc, rocfft_array_type_complex_interleaved,
+                                                rocfft_array_type_complex_interleaved, nullptr, nullptr, g.dim,
+                                                strides, (size_t)g.G, g.dim, strides, (size_t)g.G);
+    if (s != rocfft_status_success) {
+        rocfft_plan_description_destroy(desc);
+        return fft_fail(s, "data layout");
+    }
+    s = rocfft_plan_create(&p->fft_fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                           rocfft_precision_single, g.dim, lengths, (size_t)p->nc, desc);
+    if (s == rocfft_status_success)
+        s = rocfft_plan_create(&p->fft_bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                               rocfft_precision_single, g.dim, lengths, (size_t)p->nc, desc);
+    rocfft_plan_description_destroy(desc);
+    if (s != rocfft_status_success) return fft_fail(s, "plan create");
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(p->fft_fwd, &w1);
+    rocfft_plan_get_work_buffer_size(p->fft_bwd, &w2);
+    p->fft_work_size = std::max(w1, w2);
+    if ((s = rocfft_execution_info_create(&p->fft_info)) != rocfft_status_success) return fft_fail(s, "exec info");
+    if (p->fft_work_size) {
+        if (hipMalloc(&p->fft_work, p->fft_work_size) != hipSuccess)
+            return fail(GN_ENOMEM, "hipMalloc failed for the rocFFT work buffer");
+        if ((s = rocfft_execution_info_set_work_buffer(p->fft_info, p->fft_work, p->fft_work_size)) !=
+            rocfft_status_success)
+            return fft_fail(s, "work buffer");
+    }
+    if ((s = rocfft_execution_info_set_stream(p->fft_info, p->stream)) != rocfft_status_success)
+        return fft_fail(s, "set stream");
+    return GN_OK;
+}
+
+// Oversampled-grid FFT in place; forward = e^{-i}, inverse = e^{+i}, both unnormalized.
+static int run_fft(gn_plan_s *p, bool forward)
+{
+    void *buf[1] = {p->grid};
+    rocfft_status s = rocfft_execute(forward ? p->fft_fwd : p->fft_bwd, buf, nullptr, p->fft_info);
+    if (s != rocfft_status_success) return fft_fail(s, "execute");
+    return GN_OK;
+}
+
+} // namespace gn
+
+using namespace gn;
+
+extern "C" {
+
+int gn_version(void) { return GN_VERSION; }
+
+const char *gn_last_error(void) { return g_err.c_str(); }
+
+int gn_grid_params(int dim, const int64_t *modes, double eps, int max_nspread, int *rat, int *nspread, int64_t *nf,
+                   double *tau)
+{
+    g_err.clear();
+    if ((dim != 2 && dim != 3) || !modes || !rat || !nspread || !nf || !tau)
+        return fail(GN_EINVAL, "gn_grid_params: bad arguments");
+    int rc = params_from_eps(dim, modes, eps, max_nspread, rat, nspread);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a) nf[a] = a < dim ? std::max<int64_t>((int64_t)*rat * modes[a], 2 * *nspread) : 1;
+    *tau = tau_of(*rat, *nspread, modes[0]);
+    return GN_OK;
+}
+
+int gn_plan_create(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,
+                   void *stream, gn_plan **plan)
+{
+    g_err.clear();
+    if (!plan) return fail(GN_EINVAL, "plan pointer is null");
+    *plan = nullptr;
+    if (dim != 2 && dim != 3) return fail(GN_EINVAL, "dim must be 2 or 3");
+    if (!modes) return fail(GN_EINVAL, "modes is null");
+    if (rat < 2) return fail(GN_EINVAL, "rat must be >= 2");
+    const int max_ns = dim == 3 ? 8 : 16;
+    if (nspread < 2 || nspread > max_ns) return fail(GN_EINVAL, "nspread out of the supported range");
+    if (ncoils < 1) return fail(GN_EINVAL, "ncoils must be >= 1");
+    if (layout != GN_COIL_LAST && layout != GN_COIL_MAJOR) return fail(GN_EINVAL, "unknown layout");
+    for (int a = 0; a < dim; ++a)
+        if (modes[a] < 1) return fail(GN_EINVAL, "modes must be >= 1");
+
+    gn_plan_s *p = new gn_plan_s();
+    Geometry &g = p->geo;
+    g.dim = dim;
+    g.ns = nspread;
+    g.G = 1;
+    g.N = 1;
+    for (int a = 0; a < 3; ++a) {
+        g.modes[a] = a < dim ? modes[a] : 1;
+        g.nf[a] = a < dim ? std::max<int64_t>((int64_t)rat * modes[a], 2 * nspread) : 1;
+        g.G *= g.nf[a];
+        g.N *= g.modes[a];
+    }
+    p->rat = rat;
+    p->iflag = iflag;
+    p->tau = tau_of(rat, nspread, g.modes[0]);
+    p->nc = ncoils;
+    p->layout = layout;
+    p->stream = (hipStream_t)stream;
+    const double pi = 3.141592653589793;
+    for (int a = 0; a < 3; ++a) {
+        const double h = 2 * pi / (double)g.nf[a];
+        g.coef[a] = (float)(1.4426950408889634 * h * h / (4.0 * p->tau));
+    }
+    int rc = setup_tiles(p);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+
+    // deapodization tables exp(tau k^2), k = n - m//2, per axis (fft/nufft_func.py:735-738)
+    std::vector<float> deap;
+    for (int a = 0; a < 3; ++a)
+        for (int64_t n = 0; n < g.modes[a]; ++n) {
+            const double k = (double)(n - g.modes[a] / 2);
+            deap.push_back(a < dim ? (float)std::exp(p->tau * k * k) : 1.0f);
+        }
+    if (hipMalloc((void **)&p->deap, deap.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&p->err_flag, sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&p->grid, (size_t)g.G * ncoils * sizeof(float2)) != hipSuccess) {
+        gn_plan_destroy(p);
+        return fail(GN_ENOMEM, "hipMalloc failed for the oversampled grid");
+    }
+    hipError_t e = hipMemcpy(p->deap, deap.data(), deap.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        gn_plan_destroy(p);
+        return hip_fail(e, "deapodization upload");
+    }
+    rc = create_fft(p);
+    if (rc) {
+        std::string keep = g_err;
+        gn_plan_destroy(p);
+        g_err = keep;
+        return rc;
+    }
+    *plan = p;
+    return GN_OK;
+}
+
+int gn_plan_set_stream(gn_plan *p, void *stream)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    p->stream = (hipStream_t)stream;
+    if (p->fft_info) {
+        rocfft_status s = rocfft_execution_info_set_stream(p->fft_info, p->stream);
+        if (s != rocfft_status_success) return fft_fail(s, "set stream");
+    }
+    return GN_OK;
+}
+
+int gn_plan_set_tiles(gn_plan *p, const int *tile)
+{
+    g_err.clear();
+    if (!p || !tile) return fail(GN_EINVAL, "bad arguments");
+    int old[3];
+    memcpy(old, p->tile_override, sizeof(old));
+    for (int a = 0; a < 3; ++a) p->tile_override[a] = tile[a] > 0 ? tile[a] : 0;
+    Geometry save = p->geo;
+    int rc = setup_tiles(p);
+    if (rc) {
+        memcpy(p->tile_override, old, sizeof(old));
+        p->geo = save;
+        return rc;
+    }
+    if (p->has_points && (save.K != p->geo.K || memcmp(save.tile, p->geo.tile, sizeof(save.tile)) != 0)) {
+        // records are binned for the old tiles: require a new set_points
+        p->has_points = false;
+    }
+    return GN_OK;
+}
+
+int gn_set_points(gn_plan *p, int64_t M, const void *x, const void *y, const void *z, int coord_f64, double df,
+                  const float *dcf)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    if (M < 0 || M > (int64_t)INT32_MAX) return fail(GN_EINVAL, "sample count out of range");
+    if (M > 0 && (!x || !y || (p->geo.dim == 3 && !z))) return fail(GN_EINVAL, "coordinate pointer is null");
+    if (!std::isfinite(df)) return fail(GN_EINVAL, "df must be finite");
+    p->has_points = false;
+    int rc = bin_and_sort(p, M, x, y, z, coord_f64, df, dcf);
+    if (rc) return rc;
+    p->M = M;
+    p->has_dcf = dcf != nullptr;
+    p->has_points = true;
+    return GN_OK;
+}
+
+int gn_execute(gn_plan *p, int kind, const void *in, void *out)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called before gn_execute");
+    if (kind != GN_TYPE1 && kind != GN_TYPE2 && kind != GN_ADJOINT && kind != GN_TYPE21)
+        return fail(GN_EINVAL, "unknown transform kind");
+    const Geometry &g = p->geo;
+    const bool empty_in = (kind == GN_TYPE2 || kind == GN_TYPE21) ? false : p->M == 0;
+    if ((!in && !empty_in) || (!out && !(kind == GN_TYPE2 && p->M == 0)))
+        return fail(GN_EINVAL, "null data pointer");
+    if ((kind == GN_TYPE1 || kind == GN_TYPE21) && p->M == 0)
+        return fail(GN_EINVAL, "type-1 transform of zero samples (1/M undefined)");
+    const double pi = 3.141592653589793;
+    const double c = std::pow(pi / p->tau, g.dim / 2.0);
+    // type 2 uses e^{-ik.x} for iflag >= 0; type 1 the opposite sign
+    const bool fwd_dir_type2 = p->iflag >= 0;
+    int rc = GN_OK;
+    auto type2 = [&](const void *img, void *samples) -> int {
+        int r = launch_pad(p, (const float2 *)img, (float)(c / (double)g.G));
+        if (!r) r = run_fft(p, fwd_dir_type2);
+        if (!r) r = launch_gather(p, (float2 *)samples);
+        return r;
+    };
+    auto type1 = [&](const void *samples, void *img, bool dcf, double extra) -> int {
+        int r = launch_spread(p, (const float2 *)samples, dcf ? 1 : 0);
+        if (!r) r = run_fft(p, !fwd_dir_type2);
+        if (!r) r = launch_crop(p, (float2 *)img, (float)(c * extra / (double)g.G));
+        return r;
+    };
+    switch (kind) {
+    case GN_TYPE2:
+        rc = type2(in, out);
+        break;
+    case GN_TYPE1:
+        rc = type1(in, out, true, 1.0 / (double)p->M);
+        break;
+    case GN_ADJOINT:
+        rc = type1(in, out, false, 1.0);
+        break;
+    case GN_TYPE21: {
+        const int64_t need = p->M * p->nc;
+        if (need > p->ybuf_cap) {
+            if (p->ybuf) hipFree(p->ybuf);
+            p->ybuf = nullptr;
+            if (hipMalloc((void **)&p->ybuf, (size_t)need * sizeof(float2)) != hipSuccess)
+                return fail(GN_ENOMEM, "hipMalloc failed for the type-21 sample buffer");
+            p->ybuf_cap = need;
+        }
+        rc = type2(in, p->ybuf);
+        if (!rc) rc = type1(p->ybuf, out, true, 1.0 / (double)p->M);
+        break;
+    }
+    }
+    return rc;
+}
+
+int gn_plan_info(const gn_plan *p, int64_t *nf, int *nspread, double *tau, int64_t *M, int64_t *ncoils, int *tile)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    for (int a = 0; a < 3; ++a) {
+        if (nf) nf[a] = p->geo.nf[a];
+        if (tile) tile[a] = p->geo.tile[a];
+    }
+    if (nspread) *nspread = p->geo.ns;
+    if (tau) *tau = p->tau;
+    if (M) *M = p->M;
+    if (ncoils) *ncoils = p->nc;
+    return GN_OK;
+}
+
+int gn_plan_destroy(gn_plan *p)
+{
+    if (!p) return GN_OK;
+    destroy_fft(p);
+    hipFree(p->rec.cell);
+    hipFree(p->rec.frac);
+    hipFree(p->rec.bin_start);
+    hipFree(p->grid);
+    hipFree(p->ybuf);
+    hipFree(p->deap);
+    hipFree(p->err_flag);
+    delete p;
+    return GN_OK;
+}
+
+} // extern "C"

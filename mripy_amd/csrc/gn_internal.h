@@ -1,0 +1,87 @@
+// Internal definitions shared by the C-ABI layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "gaussnufft.h"
+
+namespace gn {
+
+constexpr int kMaxDim = 3;
+constexpr int kSpreadWaves = 4;     // waves (tiles) per spread workgroup
+constexpr int kGatherThreads = 256; // threads per gather workgroup
+constexpr int kCoilGroup = 8;       // coils accumulated together in the gather
+
+// Sample records in tile-sorted order: start cell of the window on each axis
+// (already wrapped to [0,nf)), the sample's original index, its offset
+// t = xi/h - m in (-1,0] on each axis and its density weight.
+struct Records {
+    int4 *cell = nullptr;   // {s0, s1, s2, original index}
+    float4 *frac = nullptr; // {t0, t1, t2, dcf}
+    int *bin_start = nullptr; // K+1 offsets of each tile's samples (home tile of s)
+    int64_t cap = 0;
+};
+
+struct Geometry {
+    int dim;
+    int64_t modes[3];
+    int64_t nf[3];
+    int64_t G;  // prod nf
+    int64_t N;  // prod modes
+    int ns;
+    int tile[3];
+    int ntiles[3];
+    int64_t K;  // number of tiles
+    float coef[3]; // log2(e) * h_a^2/(4 tau): weight = exp2(-coef * (t + ns - j)^2)
+};
+
+} // namespace gn
+
+struct gn_plan_s {
+    gn::Geometry geo;
+    int rat = 2;
+    int iflag = 1;
+    double tau = 0;
+    int64_t nc = 1;
+    int layout = GN_COIL_LAST;
+    hipStream_t stream = nullptr;
+    int tile_override[3] = {0, 0, 0};
+
+    int64_t M = 0;
+    bool has_points = false;
+    bool has_dcf = false;
+    gn::Records rec;
+
+    float2 *grid = nullptr;   // nc * G oversampled grids, coil-major
+    float2 *ybuf = nullptr;   // nc * M intermediate samples (type 21)
+    int64_t ybuf_cap = 0;
+    float *deap = nullptr;    // concatenated per-axis exp(tau k^2) tables
+    int *err_flag = nullptr;  // device flag set by set_points on bad coordinates
+
+    rocfft_plan fft_fwd = nullptr;
+    rocfft_plan fft_bwd = nullptr;
+    rocfft_execution_info fft_info = nullptr;
+    void *fft_work = nullptr;
+    size_t fft_work_size = 0;
+};
+
+namespace gn {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+// kernels.hip
+int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const void *z, int coord_f64, double df,
+                 const float *dcf);
+int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf);
+int launch_gather(gn_plan_s *p, float2 *samples);
+int launch_pad(gn_plan_s *p, const float2 *image, float scale);
+int launch_crop(gn_plan_s *p, float2 *image, float scale);
+void default_tiles(int dim, int ns, const int64_t *nf, int *tile);
+
+} // namespace gn

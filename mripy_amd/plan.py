@@ -1,0 +1,182 @@
+"""Plan: the Python owner of one C-ABI gaussnufft plan, working on torch tensors.
+
+A plan fixes the image size, coil count, kernel (rat, nspread), sign and layout;
+``set_points`` bins a trajectory; ``type2``/``type1``/``adjoint``/``type21`` run
+on the current torch stream of the plan's device.  Inputs may be torch tensors
+(device-resident, returned as device tensors, asynchronous) or numpy arrays
+(copied to the device; the result is copied back, synchronously), mirroring the
+numpy-in/numpy-out behaviour of the reference functions.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GN_ADJOINT, GN_COIL_LAST, GN_COIL_MAJOR, GN_TYPE1, GN_TYPE2, GN_TYPE21, check
+
+
+def grid_params(modes, eps, max_nspread=None):
+    """(rat, nspread, nf, tau) from eps by the reference rule (fft/nufft_func.py:198-249)."""
+    L = _lib.load()
+    dim = len(modes)
+    m = (ctypes.c_int64 * 3)(*[int(v) for v in modes] + [1] * (3 - dim))
+    rat, ns = ctypes.c_int(), ctypes.c_int()
+    nf = (ctypes.c_int64 * 3)()
+    tau = ctypes.c_double()
+    check(L.gn_grid_params(dim, m, float(eps), int(max_nspread or 0), ctypes.byref(rat), ctypes.byref(ns), nf,
+                           ctypes.byref(tau)))
+    return rat.value, ns.value, tuple(nf[:dim]), tau.value
+
+
+def _device(device):
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    return torch.device(device)
+
+
+class Plan:
+    """One NUFFT plan (C handle) bound to a device and its current stream."""
+
+    def __init__(self, modes, ncoils=1, eps=None, rat=None, nspread=None, iflag=1, layout="coil_last",
+                 device=None, max_nspread=None, tiles=None):
+        L = _lib.load()
+        self.modes = tuple(int(m) for m in modes)
+        self.dim = len(self.modes)
+        if self.dim not in (2, 3):
+            raise ValueError("only 2-D and 3-D transforms are supported")
+        if eps is not None:
+            if max_nspread is None and self.dim == 3:
+                max_nspread = 6  # every 3-D reference driver passes max_nspread=6
+            r, n, _, _ = grid_params(self.modes, eps, max_nspread)
+            rat = r if rat is None else rat
+            nspread = n if nspread is None else nspread
+        if rat is None or nspread is None:
+            raise ValueError("give eps or both rat and nspread")
+        self.rat, self.nspread = int(rat), int(nspread)
+        self.ncoils = int(ncoils)
+        self.iflag = int(iflag)
+        self.layout = GN_COIL_LAST if layout in ("coil_last", GN_COIL_LAST) else GN_COIL_MAJOR
+        self.device = _device(device)
+        self._h = ctypes.c_void_p()
+        m = (ctypes.c_int64 * 3)(*list(self.modes) + [1] * (3 - self.dim))
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            check(L.gn_plan_create(self.dim, m, self.rat, self.nspread, self.iflag, self.ncoils, self.layout,
+                                   ctypes.c_void_p(stream), ctypes.byref(self._h)))
+        if tiles is not None:
+            self.set_tiles(tiles)
+        nf = (ctypes.c_int64 * 3)()
+        ns = ctypes.c_int()
+        tau = ctypes.c_double()
+        check(L.gn_plan_info(self._h, nf, ctypes.byref(ns), ctypes.byref(tau), None, None, None))
+        self.nf = tuple(nf[:self.dim])
+        self.tau = tau.value
+        self.M = None
+        self._dcf = None
+
+    # ------------------------------------------------------------------ setup
+    def set_tiles(self, tiles):
+        t = (ctypes.c_int * 3)(*(list(tiles) + [0] * (3 - len(tiles))))
+        check(_lib.load().gn_plan_set_tiles(self._h, t))
+
+    @property
+    def tiles(self):
+        t = (ctypes.c_int * 3)()
+        check(_lib.load().gn_plan_info(self._h, None, None, None, None, None, t))
+        return tuple(t[:self.dim])
+
+    def _bind_stream(self):
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(_lib.load().gn_plan_set_stream(self._h, ctypes.c_void_p(stream)))
+
+    def set_points(self, coords, df=1.0, dcf=None):
+        """Bin sample coordinates (radians; d arrays of M).  float64 numpy inputs stay
+        float64 for the window index; torch float32 tensors are used as they are."""
+        if len(coords) != self.dim:
+            raise ValueError("need %d coordinate arrays" % self.dim)
+        ts = []
+        f64 = all(isinstance(c, np.ndarray) and c.dtype == np.float64 for c in coords) or \
+            all(torch.is_tensor(c) and c.dtype == torch.float64 for c in coords)
+        dt = torch.float64 if f64 else torch.float32
+        for c in coords:
+            t = torch.as_tensor(c).reshape(-1).to(device=self.device, dtype=dt).contiguous()
+            ts.append(t)
+        M = ts[0].numel()
+        if any(t.numel() != M for t in ts):
+            raise ValueError("coordinate arrays differ in length")
+        d = None
+        if dcf is not None:
+            d = torch.as_tensor(dcf).reshape(-1).to(device=self.device, dtype=torch.float32).contiguous()
+            if d.numel() != M:
+                raise ValueError("dcf must have one entry per sample")
+        with torch.cuda.device(self.device):
+            self._bind_stream()
+            z = ts[2].data_ptr() if self.dim == 3 else None
+            check(_lib.load().gn_set_points(self._h, M, ts[0].data_ptr(), ts[1].data_ptr(), z, 1 if f64 else 0,
+                                            float(df), d.data_ptr() if d is not None else None))
+        self.M = M
+        self._dcf = d
+        return self
+
+    # ------------------------------------------------------------------ shapes
+    def image_shape(self):
+        if self.layout == GN_COIL_LAST:
+            return self.modes + (self.ncoils,)
+        return (self.ncoils,) + self.modes
+
+    def sample_shape(self):
+        if self.layout == GN_COIL_LAST:
+            return (self.M, self.ncoils)
+        return (self.ncoils, self.M)
+
+    # ------------------------------------------------------------------ execution
+    def _run(self, kind, x, in_shape, out_shape):
+        if self.M is None:
+            raise RuntimeError("set_points must be called first")
+        is_np = not torch.is_tensor(x)
+        t = torch.as_tensor(x)
+        n_in = int(np.prod(in_shape))
+        if t.numel() != n_in:
+            raise ValueError("input has %d elements, expected %s" % (t.numel(), in_shape))
+        t = t.to(device=self.device, dtype=torch.complex64).reshape(in_shape).contiguous()
+        out = torch.empty(out_shape, dtype=torch.complex64, device=self.device)
+        with torch.cuda.device(self.device):
+            self._bind_stream()
+            check(_lib.load().gn_execute(self._h, kind, t.data_ptr() if t.numel() else None,
+                                         out.data_ptr() if out.numel() else None))
+        if is_np:
+            return out.cpu().numpy()
+        return out
+
+    def type2(self, image):
+        """Forward (reference type 2): image -> samples."""
+        return self._run(GN_TYPE2, image, self.image_shape(), self.sample_shape())
+
+    def type1(self, samples):
+        """Reference type 1 with density weighting (if a dcf was set): samples -> image."""
+        return self._run(GN_TYPE1, samples, self.sample_shape(), self.image_shape())
+
+    def adjoint(self, samples):
+        """Exact adjoint of type2 (no 1/M, no dcf)."""
+        return self._run(GN_ADJOINT, samples, self.sample_shape(), self.image_shape())
+
+    def type21(self, image):
+        """type1(dcf * type2(image)): the normal operator of the reference."""
+        return self._run(GN_TYPE21, image, self.image_shape(), self.image_shape())
+
+    def execute_into(self, kind, src, dst):
+        """Run on preallocated contiguous complex64 device tensors (no allocation, no checks of shape)."""
+        check(_lib.load().gn_execute(self._h, kind, src.data_ptr(), dst.data_ptr()))
+        return dst
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().gn_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = ctypes.c_void_p()

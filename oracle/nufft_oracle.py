@@ -1,0 +1,360 @@
+"""FP64 NumPy restatement of mripy's Gaussian-gridding NUFFT -- TEST INFRASTRUCTURE ONLY.
+
+This module is the oracle that the HIP implementation in ``mripy_amd`` is checked
+against.  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg
+of ``bench.py`` may import it, and only as the checker: the product path never
+calls into it.
+
+Every function restates the reference algorithm in float64/complex128 and cites
+the reference line it follows (paths relative to the mripy repository root).
+
+Conventions (all exact in the reference):
+
+* grid parameters: ``fft/nufft_func.py:198-249``;
+* window: ``xi = (df*x) mod 2pi`` (Python float ``%``), ``m = 1 + xi // h``,
+  taps ``g in [m-ns, m+ns)`` wrapped mod ``nf``, weight
+  ``exp(-(xi - h*g)^2 / (4 tau))`` (``fft/nufft_func.py:529-547``);
+* type 2 (forward): deapodize by ``(pi/tau)^(d/2) exp(tau*|k|^2)``, place at
+  ``k mod nf``, unnormalized forward FFT, gather, divide by ``G = prod(nf)``
+  (``fft/nufft_func.py:741-777, 1209-1258``);
+* type 1 (adjoint): spread, ``ifftn`` (normalized by 1/G), crop, multiply by
+  ``(1/M)(pi/tau)^(d/2) exp(tau*|k|^2)`` (``fft/nufft_func.py:702-738,1167-1206``);
+* ``iflag < 0`` swaps the FFT directions with the same overall scaling
+  (``fft/nufft_func.py:723-726, 768-771``).
+
+Pinning (see tests/test_oracle.py): type 1 against the brute-force NUDFT
+(``nudft*d1``, the reference's own known-answer test at its tolerance and far
+tighter), type 2 by exact adjointness with type 1 and against a brute-force
+type-2 DFT, type 21 by composition, the fast E1*E2^k*E3 weights against the
+direct weights, the Cartesian-lattice special case against ``numpy.fft``,
+periodicity, and single-sample closed forms.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TWO_PI = 2 * np.pi
+
+
+# ----------------------------------------------------------------------------
+# parameters and frequency grids
+# ----------------------------------------------------------------------------
+
+def grid_params(modes, eps=None, rat=None, nspread=None, max_nspread=None):
+    """(nspread, nf tuple, tau, rat) for image ``modes`` (fft/nufft_func.py:198-249).
+
+    Either ``eps`` (reference rule) or explicit ``rat``/``nspread`` is given.
+    ``tau`` uses the first mode count ``ms`` for every axis, as the reference
+    does (``tau = pi*lambda/ms**2``, :208,:225,:248).
+    """
+    modes = tuple(int(m) for m in modes)
+    if eps is not None:
+        if eps <= 1e-33 or eps >= 1e-1:
+            raise ValueError("eps = {0:.0e}; must satisfy 1e-33 < eps < 1e-1.".format(eps))
+        r = 2 if eps > 1e-11 else 3
+        ns = int(-np.log(eps) / (np.pi * (r - 1) / (r - 0.5)) + 0.5)
+        if max_nspread is not None:           # 3-D drivers pass max_nspread=6 (:1169)
+            ns = min(ns, max_nspread)
+            ns = max(ns, 3)
+        rat = r if rat is None else rat
+        nspread = ns if nspread is None else nspread
+    if rat is None or nspread is None:
+        raise ValueError("need eps or both rat and nspread")
+    nf = tuple(max(rat * m, 2 * nspread) for m in modes)
+    lam = nspread / (rat * (rat - 0.5))
+    tau = np.pi * lam / modes[0] ** 2
+    return nspread, nf, tau, rat
+
+
+def freqs(modes, df=1.0):
+    """Integer frequency grids ``-(m//2) .. m-(m//2)-1`` (fft/nufft_func.py:54-56,104-106,162-167)."""
+    sl = tuple(slice(-(m // 2), m - (m // 2)) for m in modes)
+    return df * np.mgrid[sl]
+
+
+def deapod_1d(m, tau):
+    """exp(tau*k^2) for k = -(m//2) .. ; the per-axis factor of the deconvolution."""
+    k = np.arange(-(m // 2), m - (m // 2), dtype=np.float64)
+    return np.exp(tau * k * k)
+
+
+# ----------------------------------------------------------------------------
+# brute-force transforms (closed-form oracles)
+# ----------------------------------------------------------------------------
+
+def nudft_type1(coords, c, modes, df=1.0, iflag=1):
+    """Brute force ``(1/M) sum_j c_j exp(+-i k.x_j)`` (fft/nufft_func.py:41-44,82-90,136-146).
+
+    ``coords``: sequence of d arrays (M,); ``c``: (M,) or (M, nc).
+    Returns shape ``modes`` (+ (nc,)).
+    """
+    sign = -1.0 if iflag < 0 else 1.0
+    ks = freqs(modes, df)
+    M = coords[0].shape[0]
+    ph = np.zeros((M,) + tuple(modes))
+    for a, x in enumerate(coords):
+        ph += x.reshape((M,) + (1,) * len(modes)) * ks[a][None]
+    E = np.exp(sign * 1j * ph)                               # (M, *modes)
+    c2 = c.reshape(M, -1)
+    out = np.tensordot(c2.T, E, axes=(1, 0))                  # (nc, *modes)
+    out = np.moveaxis(out, 0, -1) / M
+    return out.reshape(tuple(modes) + c.shape[1:])
+
+
+def nudft_type2(coords, F, modes, df=1.0, iflag=1):
+    """Brute force ``y_j = sum_k F[k] exp(-+i k.x_j)`` (the sign opposite to type 1)."""
+    sign = 1.0 if iflag < 0 else -1.0
+    ks = freqs(modes, df)
+    M = coords[0].shape[0]
+    ph = np.zeros((M,) + tuple(modes))
+    for a, x in enumerate(coords):
+        ph += x.reshape((M,) + (1,) * len(modes)) * ks[a][None]
+    E = np.exp(sign * 1j * ph).reshape(M, -1)
+    F2 = F.reshape(int(np.prod(modes)), -1)
+    out = E @ F2
+    return out.reshape((M,) + F.shape[len(modes):])
+
+
+# ----------------------------------------------------------------------------
+# window (cell index + weights)
+# ----------------------------------------------------------------------------
+
+def cell_index(x, nf):
+    """``xi = x mod 2pi`` and ``m = 1 + xi // h`` with Python float semantics.
+
+    Follows ``fft/nufft_func.py:293-295,537-540``.  ``np.mod`` and
+    ``np.floor_divide`` on float64 implement the same fmod-based divmod as
+    CPython/numba float ``%`` and ``//``.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    xi = np.mod(x, TWO_PI)
+    h = TWO_PI / nf
+    m = 1 + np.floor_divide(xi, h).astype(np.int64)
+    return xi, m
+
+
+def window_direct(x, nf, ns, tau):
+    """Direct weights ``exp(-0.25 (xi - h g)^2 / tau)`` for g in [m-ns, m+ns).
+
+    Returns (idx (M, 2ns) wrapped mod nf, w (M, 2ns)); fft/nufft_func.py:529-547.
+    """
+    xi, m = cell_index(x, nf)
+    h = TWO_PI / nf
+    g = m[:, None] + np.arange(-ns, ns)[None, :]
+    w = np.exp(-0.25 * (xi[:, None] - h * g) ** 2 / tau)
+    return np.mod(g, nf), w
+
+
+def window_fast(x, nf, ns, tau):
+    """Same weights through the fast-Gaussian-gridding factorisation E1*E2^k*E3[k].
+
+    Restates ``build_grid_1d1_fast`` (fft/nufft_func.py:302-323) for one axis:
+    ``E1 = exp(-delta^2/(4 tau))``, ``E2 = exp(delta pi/(nf tau))``,
+    ``E3[k] = exp(-(pi k/nf)^2/tau)``, ``delta = xi - h m``; tap ``m+k`` gets
+    ``E1 E2^k E3[k]`` and tap ``m-k-1`` gets ``E1 / E2^(k+1) E3[k+1]``.
+    """
+    xi, m = cell_index(x, nf)
+    h = TWO_PI / nf
+    d = xi - h * m
+    E1 = np.exp(-0.25 * d ** 2 / tau)
+    E2 = np.exp(d * np.pi / (nf * tau))
+    E3 = np.exp(-(np.pi * np.arange(ns + 1) / nf) ** 2 / tau)
+    w = np.zeros((x.shape[0], 2 * ns))
+    E2k = np.ones_like(d)
+    for k in range(ns):
+        w[:, ns + k] = E1 * E2k * E3[k]              # tap m + k
+        E2k = E2k * E2
+        w[:, ns - k - 1] = E1 / E2k * E3[k + 1]      # tap m - k - 1
+    g = m[:, None] + np.arange(-ns, ns)[None, :]
+    return np.mod(g, nf), w
+
+
+def _tensor_window(coords, nf, ns, tau, fast=False):
+    """Flattened grid indices (M, (2ns)^d) and separable weights for d axes."""
+    win = window_fast if fast else window_direct
+    M = coords[0].shape[0]
+    flat = np.zeros((M, 1), dtype=np.int64)
+    wt = np.ones((M, 1))
+    for a, x in enumerate(coords):
+        idx, w = win(x, nf[a], ns, tau)
+        flat = (flat[:, :, None] * nf[a] + idx[:, None, :]).reshape(M, -1)
+        wt = (wt[:, :, None] * w[:, None, :]).reshape(M, -1)
+    return flat, wt
+
+
+# ----------------------------------------------------------------------------
+# gridding kernels (spread = type-1 grid, gather = type-2 grid)
+# ----------------------------------------------------------------------------
+
+def spread(coords, c, nf, ns, tau, fast=False, chunk=4096):
+    """Type-1 gridding onto the oversampled grid, coil-last ``(*nf, nc)``.
+
+    Restates build_grid_2d1/3d1 (fft/nufft_func.py:529-547, 836-860) and the
+    fast variants (:550-582, :863-909); coils are independent (:931-937).
+    """
+    M = coords[0].shape[0]
+    c2 = np.asarray(c, dtype=np.complex128).reshape(M, -1)
+    nc = c2.shape[1]
+    G = int(np.prod(nf))
+    grid = np.zeros((G, nc), dtype=np.complex128)
+    for s in range(0, M, chunk):
+        e = min(M, s + chunk)
+        flat, wt = _tensor_window([x[s:e] for x in coords], nf, ns, tau, fast)
+        for t in range(nc):
+            contrib = wt * c2[s:e, t][:, None]
+            grid[:, t] += np.bincount(flat.ravel(), weights=contrib.real.ravel(), minlength=G) \
+                + 1j * np.bincount(flat.ravel(), weights=contrib.imag.ravel(), minlength=G)
+    return grid.reshape(tuple(nf) + (nc,))
+
+
+def gather(coords, grid, nf, ns, tau, fast=False, chunk=4096):
+    """Type-2 gridding: ``c_j = (1/G) sum_g w_g grid[g]`` (fft/nufft_func.py:585-639, 957-1032)."""
+    M = coords[0].shape[0]
+    G = int(np.prod(nf))
+    g2 = grid.reshape(G, -1)
+    out = np.zeros((M, g2.shape[1]), dtype=np.complex128)
+    for s in range(0, M, chunk):
+        e = min(M, s + chunk)
+        flat, wt = _tensor_window([x[s:e] for x in coords], nf, ns, tau, fast)
+        out[s:e] = np.einsum('mt,mtc->mc', wt, g2[flat])
+    return out / G
+
+
+# ----------------------------------------------------------------------------
+# pad / crop between image and oversampled grid
+# ----------------------------------------------------------------------------
+
+def _wrap_index(m, nf):
+    """Grid index of frequency k = n - m//2 for n = 0..m-1: ``k mod nf``."""
+    return np.mod(np.arange(m) - m // 2, nf)
+
+
+def place(Fk, nf):
+    """Zero-padded placement of ``Fk`` (*modes, nc) at ``k mod nf`` (fft/nufft_func.py:761-764,1226-1233)."""
+    d = len(nf)
+    modes = Fk.shape[:d]
+    out = np.zeros(tuple(nf) + Fk.shape[d:], dtype=np.complex128)
+    out[np.ix_(*[_wrap_index(m, n) for m, n in zip(modes, nf)])] = Fk
+    return out
+
+
+def crop(F, modes):
+    """Inverse of ``place``: concatenate([F[-(m//2):], F[:m//2+m%2]]) per axis (fft/nufft_func.py:728-729)."""
+    d = len(modes)
+    nf = F.shape[:d]
+    return F[np.ix_(*[_wrap_index(m, n) for m, n in zip(modes, nf)])]
+
+
+def deapod(modes, tau):
+    """``(pi/tau)^(d/2) exp(tau |k|^2)`` on the image grid (separable product)."""
+    d = len(modes)
+    out = np.array((np.pi / tau) ** (d / 2.0))
+    for m in modes:
+        out = np.multiply.outer(out, deapod_1d(m, tau))
+    return out
+
+
+# ----------------------------------------------------------------------------
+# drivers
+# ----------------------------------------------------------------------------
+
+class Params:
+    """Resolved transform parameters shared by the drivers."""
+
+    def __init__(self, modes, eps=None, rat=None, nspread=None, df=1.0, iflag=1, max_nspread=None):
+        self.modes = tuple(int(m) for m in modes)
+        self.ns, self.nf, self.tau, self.rat = grid_params(self.modes, eps, rat, nspread, max_nspread)
+        self.df = float(df)
+        self.iflag = int(iflag)
+        self.G = int(np.prod(self.nf))
+        self.d = len(self.modes)
+
+
+def _as_coil_last(a, lead):
+    """View ``a`` as (*lead, nc) and report whether a coil axis was present."""
+    a = np.asarray(a)
+    has_coil = a.ndim > len(lead)
+    return a.reshape(tuple(lead) + (-1,)), has_coil
+
+
+def type1(coords, c, p: Params, fast=False):
+    """Adjoint NUFFT (reference type 1): samples (M[,nc]) -> image (*modes[,nc])."""
+    coords = [np.asarray(x, dtype=np.float64) * p.df for x in coords]
+    M = coords[0].shape[0]
+    c2, has_coil = _as_coil_last(c, (M,))
+    grid = spread(coords, c2, p.nf, p.ns, p.tau, fast)
+    axes = tuple(range(p.d))
+    if p.iflag < 0:
+        F = np.fft.fftn(grid, axes=axes) / p.G
+    else:
+        F = np.fft.ifftn(grid, axes=axes)
+    F = crop(F, p.modes)
+    out = F * deapod(p.modes, p.tau)[..., None] / M
+    return out if has_coil else out[..., 0]
+
+
+def type2(coords, Fk, p: Params, fast=False):
+    """Forward NUFFT (reference type 2): image (*modes[,nc]) -> samples (M[,nc])."""
+    coords = [np.asarray(x, dtype=np.float64) * p.df for x in coords]
+    F2, has_coil = _as_coil_last(Fk, p.modes)
+    F2 = F2 * deapod(p.modes, p.tau)[..., None]
+    grid = place(F2, p.nf)
+    axes = tuple(range(p.d))
+    if p.iflag < 0:
+        grid = np.fft.ifftn(grid, axes=axes) * p.G
+    else:
+        grid = np.fft.fftn(grid, axes=axes)
+    out = gather(coords, grid, p.nf, p.ns, p.tau, fast)
+    return out if has_coil else out[:, 0]
+
+
+def type21(coords, Fk, p: Params, dcf=None, fast=False):
+    """Normal operator ``type1(dcf * type2(Fk))`` (fft/nufft_func.py:780-830, 1058-1106, 1261-1330)."""
+    y = type2(coords, Fk, p, fast)
+    if dcf is not None:
+        dcf = np.asarray(dcf, dtype=np.float64).reshape(-1)
+        y = y * (dcf if y.ndim == 1 else dcf[:, None])
+    return type1(coords, y, p, fast)
+
+
+def adjoint_exact(coords, y, p: Params, fast=False):
+    """Exact adjoint of ``type2``: ``M * type1`` (no density weighting)."""
+    M = np.asarray(coords[0]).shape[0]
+    return type1(coords, y, p, fast) * M
+
+
+# ----------------------------------------------------------------------------
+# reference function names (fft/nufft_func.py:702,741,780,1167,1209,1261)
+# ----------------------------------------------------------------------------
+
+def nufft2d1_gaussker(x, y, c, ms, mt, df=1.0, eps=1e-15, iflag=1, gridfast=1, **kw):
+    p = Params((ms, mt), eps=eps, df=df, iflag=iflag, **kw)
+    return type1([x, y], c, p, fast=bool(gridfast))
+
+
+def nufft2d2_gaussker(x, y, Fk, ms, mt, df=1.0, eps=1e-15, iflag=1, gridfast=1, **kw):
+    p = Params((ms, mt), eps=eps, df=df, iflag=iflag, **kw)
+    return type2([x, y], Fk, p, fast=bool(gridfast))
+
+
+def nufft2d21_gaussker(x, y, Fk, ms, mt, df=1.0, eps=1e-15, iflag=1, gridfast=1, **kw):
+    p = Params((ms, mt), eps=eps, df=df, iflag=iflag, **kw)
+    return type21([x, y], Fk, p, None, fast=bool(gridfast))
+
+
+def nufft3d1_gaussker(x, y, z, c, ms, mt, mu, df=1.0, eps=1e-15, iflag=1, gridfast=1, **kw):
+    kw.setdefault('max_nspread', 6)
+    p = Params((ms, mt, mu), eps=eps, df=df, iflag=iflag, **kw)
+    return type1([x, y, z], c, p, fast=bool(gridfast))
+
+
+def nufft3d2_gaussker(x, y, z, Fk, ms, mt, mu, df=1.0, eps=1e-15, iflag=1, gridfast=0, **kw):
+    kw.setdefault('max_nspread', 6)
+    p = Params((ms, mt, mu), eps=eps, df=df, iflag=iflag, **kw)
+    return type2([x, y, z], Fk, p, fast=bool(gridfast))
+
+
+def nufft3d21_gaussker(x, y, z, Fk, ms, mt, mu, dcf=None, df=1.0, eps=1e-15, iflag=1, gridfast=0, **kw):
+    kw.setdefault('max_nspread', 6)
+    p = Params((ms, mt, mu), eps=eps, df=df, iflag=iflag, **kw)
+    return type21([x, y, z], Fk, p, dcf, fast=bool(gridfast))

@@ -94,13 +94,30 @@ int gn_set_points(gn_plan *plan, int64_t M, const void *x, const void *y, const 
  * stream.  in/out: device complex64 buffers in the plan's layout.  Asynchronous. */
 int gn_execute(gn_plan *plan, int kind, const void *in, void *out);
 
-/* Query: nf[3] oversampled grid, nspread, tau, number of samples, coils, tile shape. */
+/* Query: nf[3] oversampled grid, nspread, tau, number of samples, coils, and
+ * tile[6] = spread (output) tile shape followed by the gather (home) tile shape. */
 int gn_plan_info(const gn_plan *plan, int64_t *nf, int *nspread, double *tau, int64_t *M, int64_t *ncoils,
                  int *tile);
 
-/* Override the spread/gather tile shape (cells per axis; 0 keeps the default).
- * Takes effect at the next gn_set_points. */
+/* Override the tile shapes, tile[6] = spread tile (3 axes) then gather tile (3 axes),
+ * in cells; 0 keeps the default.  Invalidates the binned points: call gn_set_points
+ * again before gn_execute. */
 int gn_plan_set_tiles(gn_plan *plan, const int *tile);
+
+/* Stage timers (replaces the wall-clock utc.timing prints around the gridding and FFT
+ * stages, utilities/utilities_class.py:30-60, fft/nufft_func_cuda.py:1472-1496).
+ * When enabled, gn_execute records a HIP event pair around every stage on the plan's
+ * stream.  gn_plan_stage_times synchronizes the stream and returns, per stage, the
+ * accumulated milliseconds and launch counts (GN_STAGE_COUNT entries each). */
+#define GN_STAGE_PAD 0     /* deapodize + zero-pad + transpose (type 2)  */
+#define GN_STAGE_FFT2 1    /* oversampled FFT of type 2                  */
+#define GN_STAGE_GATHER 2  /* interpolation onto the samples (type 2)    */
+#define GN_STAGE_SPREAD 3  /* gridding of the samples (type 1)           */
+#define GN_STAGE_FFT1 4    /* oversampled FFT of type 1                  */
+#define GN_STAGE_CROP 5    /* crop + deapodize + scale + transpose       */
+#define GN_STAGE_COUNT 6
+int gn_plan_set_timing(gn_plan *plan, int enable);
+int gn_plan_stage_times(gn_plan *plan, double *ms, int64_t *counts, int reset);
 
 int gn_plan_destroy(gn_plan *plan);
 

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libgaussnufft.so")
 GN_OK, GN_EINVAL, GN_ENOMEM, GN_EHIP, GN_EROCFFT, GN_ESTATE = 0, 1, 2, 3, 4, 5
 GN_COIL_LAST, GN_COIL_MAJOR = 0, 1
 GN_TYPE1, GN_TYPE2, GN_ADJOINT, GN_TYPE21 = 1, 2, 3, 21
+STAGES = ("pad", "fft2", "gather", "spread", "fft1", "crop")
 
 # every symbol declared in include/gaussnufft.h, with its ctypes signature
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -34,6 +35,8 @@ SIGNATURES = {
     "gn_execute": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "gn_plan_info": ([_vp, _i64p, _ip, ctypes.POINTER(ctypes.c_double), _i64p, _i64p, _ip], ctypes.c_int),
     "gn_plan_set_tiles": ([_vp, _ip], ctypes.c_int),
+    "gn_plan_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+    "gn_plan_stage_times": ([_vp, ctypes.POINTER(ctypes.c_double), _i64p, ctypes.c_int], ctypes.c_int),
     "gn_plan_destroy": ([_vp], ctypes.c_int),
 }
 

@@ -68,27 +68,45 @@ static double tau_of(int rat, int ns, int64_t ms)
 static int setup_tiles(gn_plan_s *p)
 {
     Geometry &g = p->geo;
-    int tile[3] = {1, 1, 1};
-    default_tiles(g.dim, g.ns, g.nf, tile);
-    for (int a = 0; a < g.dim; ++a)
-        if (p->tile_override[a] > 0) tile[a] = (int)std::min<int64_t>(p->tile_override[a], g.nf[a]);
+    int tile[3] = {1, 1, 1}, gtile[3] = {1, 1, 1};
+    default_tiles(g.dim, g.ns, g.nf, tile, gtile);
+    for (int a = 0; a < g.dim; ++a) {
+        if (p->tile_override[a] > 0) tile[a] = p->tile_override[a];
+        if (p->gtile_override[a] > 0) gtile[a] = p->gtile_override[a];
+    }
+    // a window must not leave a tile and re-enter it around the periodic axis:
+    // otherwise one tile spans the whole axis
+    auto fit = [&](int t, int a) {
+        t = (int)std::min<int64_t>(std::max(t, 1), g.nf[a]);
+        if (t + 2 * g.ns - 1 > g.nf[a]) t = (int)g.nf[a];
+        return t;
+    };
     g.K = 1;
+    g.Kg = 1;
     for (int a = 0; a < 3; ++a) {
         if (a >= g.dim) {
-            g.tile[a] = 1;
-            g.ntiles[a] = 1;
+            g.tile[a] = g.gtile[a] = 1;
+            g.ntiles[a] = g.ngtiles[a] = 1;
             continue;
         }
-        g.tile[a] = tile[a];
-        g.ntiles[a] = (int)((g.nf[a] + tile[a] - 1) / tile[a]);
+        g.tile[a] = fit(tile[a], a);
+        g.gtile[a] = fit(gtile[a], a);
+        g.ntiles[a] = (int)((g.nf[a] + g.tile[a] - 1) / g.tile[a]);
+        g.ngtiles[a] = (int)((g.nf[a] + g.gtile[a] - 1) / g.gtile[a]);
         g.K *= g.ntiles[a];
+        g.Kg *= g.ngtiles[a];
     }
-    // LDS limits: spread keeps kSpreadWaves tiles of float4, gather stages tile + halo
-    int64_t tc = (int64_t)g.tile[0] * g.tile[1] * g.tile[2];
+    // LDS limits: the spread keeps per-coil planes of up to kCoilGroup coils of a tile,
+    // the gather stages tile + halo of one coil
+    const int64_t tc = (int64_t)g.tile[0] * g.tile[1] * g.tile[2];
+    const int64_t ncg = std::min<int64_t>(p->nc, kCoilGroup);
     int64_t rc = 1;
-    for (int a = 0; a < g.dim; ++a) rc *= std::min<int64_t>(g.tile[a] + 2 * g.ns - 1, g.nf[a]);
-    if (tc * 16 * kSpreadWaves > 160 * 1024 || rc * 8 > 160 * 1024)
+    for (int a = 0; a < g.dim; ++a) rc *= std::min<int64_t>(g.gtile[a] + 2 * g.ns - 1, g.nf[a]);
+    if (tc * ncg * 8 > 160 * 1024 || rc * 8 > 160 * 1024)
         return fail(GN_EINVAL, "tile shape exceeds the 160 KiB LDS budget");
+    for (int a = 0; a < g.dim; ++a)
+        if (g.tile[a] > 1024) return fail(GN_EINVAL, "spread tile extent above 1024 cells");
+    if (g.K > (int64_t)INT32_MAX / 2 || g.Kg > (int64_t)INT32_MAX / 2) return fail(GN_EINVAL, "too many tiles");
     return GN_OK;
 }
 
@@ -144,6 +162,29 @@ static int create_fft(gn_plan_s *p)
     if ((s = rocfft_execution_info_set_stream(p->fft_info, p->stream)) != rocfft_status_success)
         return fft_fail(s, "set stream");
     return GN_OK;
+}
+
+// Stage timer: records an event pair around fn() when timing is enabled.
+template <typename F>
+static int timed(gn_plan_s *p, int stage, F fn)
+{
+    if (!p->timing) return fn();
+    auto take = [&]() -> hipEvent_t {
+        if (!p->event_pool.empty()) {
+            hipEvent_t e = p->event_pool.back();
+            p->event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hipEventCreate(&e);
+        return e;
+    };
+    hipEvent_t a = take(), b = take();
+    hipEventRecord(a, p->stream);
+    int rc = fn();
+    hipEventRecord(b, p->stream);
+    p->pending.push_back({stage, a, b});
+    return rc;
 }
 
 // Oversampled-grid FFT in place; forward = e^{-i}, inverse = e^{+i}, both unnormalized.
@@ -268,17 +309,23 @@ int gn_plan_set_tiles(gn_plan *p, const int *tile)
 {
     g_err.clear();
     if (!p || !tile) return fail(GN_EINVAL, "bad arguments");
-    int old[3];
+    int old[3], oldg[3];
     memcpy(old, p->tile_override, sizeof(old));
-    for (int a = 0; a < 3; ++a) p->tile_override[a] = tile[a] > 0 ? tile[a] : 0;
+    memcpy(oldg, p->gtile_override, sizeof(oldg));
+    for (int a = 0; a < 3; ++a) {
+        p->tile_override[a] = tile[a] > 0 ? tile[a] : 0;
+        p->gtile_override[a] = tile[3 + a] > 0 ? tile[3 + a] : 0;
+    }
     Geometry save = p->geo;
     int rc = setup_tiles(p);
     if (rc) {
         memcpy(p->tile_override, old, sizeof(old));
+        memcpy(p->gtile_override, oldg, sizeof(oldg));
         p->geo = save;
         return rc;
     }
-    if (p->has_points && (save.K != p->geo.K || memcmp(save.tile, p->geo.tile, sizeof(save.tile)) != 0)) {
+    if (p->has_points && (memcmp(save.tile, p->geo.tile, sizeof(save.tile)) != 0 ||
+                          memcmp(save.gtile, p->geo.gtile, sizeof(save.gtile)) != 0)) {
         // records are binned for the old tiles: require a new set_points
         p->has_points = false;
     }
@@ -321,15 +368,17 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
     const bool fwd_dir_type2 = p->iflag >= 0;
     int rc = GN_OK;
     auto type2 = [&](const void *img, void *samples) -> int {
-        int r = launch_pad(p, (const float2 *)img, (float)(c / (double)g.G));
-        if (!r) r = run_fft(p, fwd_dir_type2);
-        if (!r) r = launch_gather(p, (float2 *)samples);
+        int r = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)img, (float)(c / (double)g.G)); });
+        if (!r) r = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, fwd_dir_type2); });
+        if (!r) r = timed(p, GN_STAGE_GATHER, [&] { return launch_gather(p, (float2 *)samples); });
         return r;
     };
     auto type1 = [&](const void *samples, void *img, bool dcf, double extra) -> int {
-        int r = launch_spread(p, (const float2 *)samples, dcf ? 1 : 0);
-        if (!r) r = run_fft(p, !fwd_dir_type2);
-        if (!r) r = launch_crop(p, (float2 *)img, (float)(c * extra / (double)g.G));
+        int r = timed(p, GN_STAGE_SPREAD, [&] { return launch_spread(p, (const float2 *)samples, dcf ? 1 : 0); });
+        if (!r) r = timed(p, GN_STAGE_FFT1, [&] { return run_fft(p, !fwd_dir_type2); });
+        if (!r)
+            r = timed(p, GN_STAGE_CROP,
+                      [&] { return launch_crop(p, (float2 *)img, (float)(c * extra / (double)g.G)); });
         return r;
     };
     switch (kind) {
@@ -365,7 +414,10 @@ int gn_plan_info(const gn_plan *p, int64_t *nf, int *nspread, double *tau, int64
     if (!p) return fail(GN_EINVAL, "plan is null");
     for (int a = 0; a < 3; ++a) {
         if (nf) nf[a] = p->geo.nf[a];
-        if (tile) tile[a] = p->geo.tile[a];
+        if (tile) {
+            tile[a] = p->geo.tile[a];
+            tile[3 + a] = p->geo.gtile[a];
+        }
     }
     if (nspread) *nspread = p->geo.ns;
     if (tau) *tau = p->tau;
@@ -374,13 +426,62 @@ int gn_plan_info(const gn_plan *p, int64_t *nf, int *nspread, double *tau, int64
     return GN_OK;
 }
 
+int gn_plan_set_timing(gn_plan *p, int enable)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    p->timing = enable != 0;
+    return GN_OK;
+}
+
+int gn_plan_stage_times(gn_plan *p, double *ms, int64_t *counts, int reset)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    if (!p->pending.empty()) {
+        hipError_t e = hipStreamSynchronize(p->stream);
+        if (e != hipSuccess) return hip_fail(e, "stage timer synchronize");
+        for (auto &q : p->pending) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, q.a, q.b) == hipSuccess) {
+                p->stage_ms[q.stage] += t;
+                p->stage_n[q.stage] += 1;
+            }
+            p->event_pool.push_back(q.a);
+            p->event_pool.push_back(q.b);
+        }
+        p->pending.clear();
+    }
+    for (int i = 0; i < GN_STAGE_COUNT; ++i) {
+        if (ms) ms[i] = p->stage_ms[i];
+        if (counts) counts[i] = p->stage_n[i];
+        if (reset) {
+            p->stage_ms[i] = 0;
+            p->stage_n[i] = 0;
+        }
+    }
+    return GN_OK;
+}
+
 int gn_plan_destroy(gn_plan *p)
 {
     if (!p) return GN_OK;
+    if (!p->pending.empty()) hipStreamSynchronize(p->stream);
+    for (auto &q : p->pending) {
+        hipEventDestroy(q.a);
+        hipEventDestroy(q.b);
+    }
+    for (auto e : p->event_pool) hipEventDestroy(e);
     destroy_fft(p);
     hipFree(p->rec.cell);
     hipFree(p->rec.frac);
     hipFree(p->rec.bin_start);
+    hipFree(p->rec.pair_a);
+    hipFree(p->rec.pair_b);
+    hipFree(p->rec.spread_items);
+    hipFree(p->rec.reduce_items);
+    hipFree(p->rec.gather_items);
+    hipFree(p->rec.partial);
     hipFree(p->grid);
     hipFree(p->ybuf);
     hipFree(p->deap);

@@ -6,24 +6,44 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "gaussnufft.h"
 
 namespace gn {
 
 constexpr int kMaxDim = 3;
-constexpr int kSpreadWaves = 4;     // waves (tiles) per spread workgroup
 constexpr int kGatherThreads = 256; // threads per gather workgroup
-constexpr int kCoilGroup = 8;       // coils accumulated together in the gather
+constexpr int kCoilGroup = 8;       // coils accumulated together per pass
+constexpr int kSpreadChunk = 1024;  // max (tile, sample) pairs per spread work item
+constexpr int kGatherChunk = 1024;  // max samples per gather work item
 
 // Sample records in tile-sorted order: start cell of the window on each axis
 // (already wrapped to [0,nf)), the sample's original index, its offset
 // t = xi/h - m in (-1,0] on each axis and its density weight.
 struct Records {
-    int4 *cell = nullptr;   // {s0, s1, s2, original index}
-    float4 *frac = nullptr; // {t0, t1, t2, dcf}
-    int *bin_start = nullptr; // K+1 offsets of each tile's samples (home tile of s)
+    int4 *cell = nullptr;     // {s0, s1, s2, original index}, sorted by gather tile
+    float4 *frac = nullptr;   // {t0, t1, t2, dcf}
+    int *bin_start = nullptr; // Kg+1 offsets of each gather tile's samples
     int64_t cap = 0;
+    // spread: one 32-byte record per (output tile, sample) pair, grouped by tile:
+    //   a = {orig, j0 | j1<<8 | j2<<16, c0 | c1<<8 | c2<<16, l0 | l1<<10 | l2<<20}
+    //   b = {t0, t1, t2, dcf} (float bits)
+    // (j = first window tap inside the tile, c = taps inside, l = tile-local cell of tap j)
+    int4 *pair_a = nullptr;
+    int4 *pair_b = nullptr;
+    int64_t npairs = 0;
+    int64_t pairs_cap = 0;
+    // work items
+    int4 *spread_items = nullptr; // {tile, pbeg, pend, slot or -1}
+    int n_spread_items = 0;
+    int4 *reduce_items = nullptr; // {tile, slot0, nslots, 0}
+    int n_reduce_items = 0;
+    int n_slots = 0;
+    int4 *gather_items = nullptr; // {gather tile, sbeg, send, 0}
+    int n_gather_items = 0;
+    float2 *partial = nullptr;    // n_slots * nc * TC partial tiles
+    int64_t partial_cap = 0;
 };
 
 struct Geometry {
@@ -33,9 +53,12 @@ struct Geometry {
     int64_t G;  // prod nf
     int64_t N;  // prod modes
     int ns;
-    int tile[3];
+    int tile[3];   // spread (output) tiles
     int ntiles[3];
-    int64_t K;  // number of tiles
+    int64_t K;     // number of spread tiles
+    int gtile[3];  // gather (home) tiles
+    int ngtiles[3];
+    int64_t Kg;    // number of gather tiles
     float coef[3]; // log2(e) * h_a^2/(4 tau): weight = exp2(-coef * (t + ns - j)^2)
 };
 
@@ -50,6 +73,7 @@ struct gn_plan_s {
     int layout = GN_COIL_LAST;
     hipStream_t stream = nullptr;
     int tile_override[3] = {0, 0, 0};
+    int gtile_override[3] = {0, 0, 0};
 
     int64_t M = 0;
     bool has_points = false;
@@ -67,6 +91,17 @@ struct gn_plan_s {
     rocfft_execution_info fft_info = nullptr;
     void *fft_work = nullptr;
     size_t fft_work_size = 0;
+
+    // optional stage timers
+    bool timing = false;
+    struct Pending {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double stage_ms[GN_STAGE_COUNT] = {0};
+    int64_t stage_n[GN_STAGE_COUNT] = {0};
 };
 
 namespace gn {
@@ -82,6 +117,6 @@ int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf);
 int launch_gather(gn_plan_s *p, float2 *samples);
 int launch_pad(gn_plan_s *p, const float2 *image, float scale);
 int launch_crop(gn_plan_s *p, float2 *image, float scale);
-void default_tiles(int dim, int ns, const int64_t *nf, int *tile);
+void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile);
 
 } // namespace gn

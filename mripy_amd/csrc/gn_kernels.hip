@@ -1,9 +1,11 @@
 // HIP kernels of the Gaussian-gridding NUFFT for CDNA4 (gfx950).
 //
 // Data layout in HBM (per plan):
-//   grid   : nc oversampled grids, coil-major, C order (nf0, nf1[, nf2]), complex64
-//   records: samples sorted by home tile (tile containing the window start cell)
-//            cell = {s0,s1,s2,orig}, frac = {t0,t1,t2,dcf}; bin_start[K+1]
+//   grid    : nc oversampled grids, coil-major, C order (nf0, nf1[, nf2]), complex64
+//   records : samples sorted by home tile (the tile holding the window start cell)
+//             cell = {s0,s1,s2,orig}, frac = {t0,t1,t2,dcf}; bin_start[K+1]
+//   pairs   : (output tile, sample) pairs grouped by tile, samples in sorted order
+//   items   : spread/gather work items (a tile and a bounded slice of its list)
 //   image / samples at the boundary: coil-last (reference) or coil-major.
 //
 // Window convention (fft/nufft_func.py:293-295, 536-546): xi = x mod 2pi,
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a)
             x = 0.0;
         }
         double xi, dummy;
-        py_divmod(x, kTwoPi, &xi);                 // xi = x % (2 pi)
+        py_divmod(x, kTwoPi, &xi);                         // xi = x % (2 pi)
         const double h = kTwoPi / (double)a.nf[ax];
         int64_t m = 1 + (int64_t)py_divmod(xi, h, &dummy); // m = 1 + int(xi // h)
         double delta = xi - h * (double)m;
@@ -96,20 +98,21 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Stable counting sort of samples by home tile.
-//   cnt[key*C + chunk]  : samples of chunk with that key (key-major)
-//   exclusive scan      : offsets, key-major so chunk order is preserved
-//   scatter             : each chunk places its samples in index order
+// Stable counting sort of items by an integer key in [0, K):
+//   cnt[key*C + chunk]  : items of chunk with that key (key-major)
+//   exclusive scan      : offsets; key-major order keeps the chunk order
+//   scatter             : each chunk places its items in index order
+// keys are read as keys[i * stride].
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_bin_count(int64_t M, const int4 *__restrict__ cell, int64_t chunk_len, int C,
-                                                   int *__restrict__ cnt)
+__global__ __launch_bounds__(256) void k_bin_count(int64_t n, const int *__restrict__ keys, int stride,
+                                                   int64_t chunk_len, int C, int *__restrict__ cnt)
 {
     const int chunk = blockIdx.x;
     const int64_t beg = (int64_t)chunk * chunk_len;
-    const int64_t end = min(M, beg + chunk_len);
+    const int64_t end = min(n, beg + chunk_len);
     for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
-        atomicAdd(&cnt[(int64_t)cell[i].w * C + chunk], 1);
+        atomicAdd(&cnt[(int64_t)keys[i * stride] * C + chunk], 1);
 }
 
 constexpr int kScanThreads = 256;
@@ -155,6 +158,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_local(int *__restrict__ d
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
+// single block: exclusive scan of the block sums; bsum[nb] receives the grand total
 __global__ __launch_bounds__(kScanThreads) void k_scan_bsums(int *__restrict__ bsum, int64_t nb)
 {
     __shared__ int lds[kScanThreads];
@@ -177,6 +181,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_bsums(int *__restrict__ b
         }
         carry += total;
     }
+    if (threadIdx.x == 0) bsum[nb] = carry;
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_add(int *__restrict__ data, int64_t n,
@@ -188,28 +193,29 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(int *__restrict__ dat
         if (base + k < n) data[base + k] += add;
 }
 
-__global__ __launch_bounds__(256) void k_bin_starts(const int *__restrict__ offs, int64_t K, int C, int64_t M,
+__global__ __launch_bounds__(256) void k_bin_starts(const int *__restrict__ offs, int64_t K, int C, int64_t n,
                                                     int *__restrict__ bin_start)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < K) bin_start[k] = offs[k * C];
-    if (k == K) bin_start[K] = (int)M;
+    if (k == K) bin_start[K] = (int)n;
 }
 
-// One wave per chunk; samples are placed group by group in index order, so
+// One wave per chunk; items are placed group by group in index order, so
 // equal keys keep their original order (stable, deterministic).
-__global__ __launch_bounds__(64) void k_bin_scatter(int64_t M, const int4 *__restrict__ cell, int64_t chunk_len, int C,
-                                                    int *__restrict__ offs, int *__restrict__ sorted)
+__global__ __launch_bounds__(64) void k_bin_scatter(int64_t n, const int *__restrict__ keys, int stride,
+                                                    int64_t chunk_len, int C, int *__restrict__ offs,
+                                                    int *__restrict__ sorted)
 {
     __shared__ int sk[64];
     const int lane = threadIdx.x;
     const int chunk = blockIdx.x;
     const int64_t beg = (int64_t)chunk * chunk_len;
-    const int64_t end = min(M, beg + chunk_len);
+    const int64_t end = min(n, beg + chunk_len);
     for (int64_t g = beg; g < end; g += 64) {
         const int64_t i = g + lane;
         const bool valid = i < end;
-        const int key = valid ? cell[i].w : -1;
+        const int key = valid ? keys[i * stride] : -1;
         sk[lane] = key;
         __syncthreads();
         int rank = 0;
@@ -247,187 +253,336 @@ __global__ __launch_bounds__(256) void k_make_records(int64_t M, const int *__re
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint spread: output-stationary LDS tiles, one wave per tile and coil.
+// (tile, sample) pair records for the output-stationary spread
+// ---------------------------------------------------------------------------
+
+struct TileGeom {
+    int dim;
+    int64_t nf[3];
+    int tile[3];
+    int ntiles[3];
+    int W; // window taps per axis (2 nspread)
+};
+
+// distinct tiles touched by the window [s, s+W) mod nf along one axis, in tap order;
+// tiles are chosen so a window never re-enters a tile it left (T + W - 1 <= nf or T = nf)
+__device__ inline int axis_tiles(int s, int W, int nf, int T, int *out)
+{
+    int n = 0, last = -1;
+    for (int j = 0; j < W; ++j) {
+        int c = s + j;
+        if (c >= nf) c -= nf;
+        const int t = c / T;
+        if (t != last) {
+            out[n++] = t;
+            last = t;
+        }
+    }
+    return n;
+}
+
+// window taps [jlo, jlo+cnt) of a window starting at s that fall in tile q, and the
+// tile-local cell of tap jlo
+__device__ inline void tap_box(int s, int q, int T, int nf, int W, int &jlo, int &cnt, int &lstart)
+{
+    const int org = q * T;
+    const int teff = min(T, nf - org);
+    if (T == nf) {
+        jlo = 0;
+        cnt = W;
+        lstart = s;
+        return;
+    }
+    int v = org - s;
+    if (v < 0) v += nf;
+    if (v < W) {
+        jlo = v;
+        cnt = min(W - v, teff);
+        lstart = 0;
+    } else {
+        jlo = 0;
+        cnt = min(W, teff - (s - org));
+        lstart = s - org;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pair_count(int64_t M, const int4 *__restrict__ rec_cell, TileGeom g,
+                                                    int *__restrict__ cnt)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= M) return;
+    const int4 rc = rec_cell[p];
+    const int s[3] = {rc.x, rc.y, rc.z};
+    int tl[32];
+    int n = 1;
+    for (int ax = 0; ax < g.dim; ++ax) n *= axis_tiles(s[ax], g.W, (int)g.nf[ax], g.tile[ax], tl);
+    cnt[p] = n;
+}
+
+__global__ __launch_bounds__(256) void k_pair_emit(int64_t M, const int4 *__restrict__ rec_cell,
+                                                   const float4 *__restrict__ rec_frac, TileGeom g,
+                                                   const int *__restrict__ offs, int *__restrict__ pkey,
+                                                   int4 *__restrict__ pa, int4 *__restrict__ pb)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= M) return;
+    const int4 rc = rec_cell[p];
+    const float4 rf = rec_frac[p];
+    int t0[32], t1[32], t2[32];
+    const int n0 = axis_tiles(rc.x, g.W, (int)g.nf[0], g.tile[0], t0);
+    const int n1 = axis_tiles(rc.y, g.W, (int)g.nf[1], g.tile[1], t1);
+    int n2 = 1;
+    t2[0] = 0;
+    if (g.dim == 3) n2 = axis_tiles(rc.z, g.W, (int)g.nf[2], g.tile[2], t2);
+    const int4 b = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
+    int o = offs[p];
+    for (int i = 0; i < n0; ++i) {
+        int j0, c0, l0;
+        tap_box(rc.x, t0[i], g.tile[0], (int)g.nf[0], g.W, j0, c0, l0);
+        for (int k = 0; k < n1; ++k) {
+            int j1, c1, l1;
+            tap_box(rc.y, t1[k], g.tile[1], (int)g.nf[1], g.W, j1, c1, l1);
+            for (int m = 0; m < n2; ++m) {
+                int j2 = 0, c2 = 1, l2 = 0;
+                int key = t0[i] * g.ntiles[1] + t1[k];
+                if (g.dim == 3) {
+                    tap_box(rc.z, t2[m], g.tile[2], (int)g.nf[2], g.W, j2, c2, l2);
+                    key = key * g.ntiles[2] + t2[m];
+                }
+                pkey[o] = key;
+                pa[o] = make_int4(rc.w, j0 | (j1 << 8) | (j2 << 16), c0 | (c1 << 8) | (c2 << 16),
+                                  l0 | (l1 << 10) | (l2 << 20));
+                pb[o] = b;
+                ++o;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_permute_pairs(int64_t n, const int *__restrict__ sorted,
+                                                       const int4 *__restrict__ ia, const int4 *__restrict__ ib,
+                                                       int4 *__restrict__ oa, int4 *__restrict__ ob)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
+        const int i = sorted[q];
+        oa[q] = ia[i];
+        ob[q] = ib[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint spread: output-stationary LDS tiles.
 //
-// A wave owns one tile of the grid for one coil.  It visits the samples whose
-// home tile is the tile itself or one of its predecessors (those are the only
-// samples whose windows can reach it), in sorted order, one sample at a time;
-// the lanes take the sample's taps.  Taps inside the tile update the tile's
-// Kahan-compensated (sum, compensation) pair in LDS.  Within one sample every
-// lane hits a distinct cell and only this wave touches this LDS region, so the
-// summation order is fixed: the result is deterministic.  The tile is written
-// to HBM once, coalesced along the fastest axis.  No global atomics.
+// One wave owns one work item: a tile and a slice of at most kSpreadChunk of
+// the (tile, sample) pair records whose windows touch it.  The 64 records of a
+// batch are loaded with coalesced loads (one per lane, with the sample's coil
+// values) and broadcast one at a time with readlane; the lanes then take the
+// taps of the window that fall inside the tile.  Each tap's weight is formed
+// once and accumulated for up to 8 coils by a read-modify-write of per-coil
+// LDS planes.  Lanes of one instruction touch distinct cells, a tile slice
+// belongs to one wave and LDS operations of a wave complete in program order,
+// so no atomics are needed and the summation order is fixed: the result is
+// deterministic.  The tile is written to HBM once, coalesced; slices of a tile
+// that was split (dense k-space centre) go to partial buffers that
+// k_spread_reduce combines with Kahan compensation in slice order.
 // ---------------------------------------------------------------------------
 
 struct SpreadArgs {
     int64_t nf[3];
     int tile[3];
     int ntiles[3];
-    int npred[3];   // candidate home tiles per axis (tile itself + predecessors)
-    int64_t K;
     int64_t G;
     int64_t M;
     int64_t nc;
+    int TC;
     int layout;
     int use_dcf;
     float coef[3];
-    const int *bin_start;
-    const int4 *rec_cell;
-    const float4 *rec_frac;
+    const int4 *items;
+    const int4 *pair_a;
+    const int4 *pair_b;
     const float2 *in;
     float2 *grid;
+    float2 *partial;
 };
 
-template <int D>
-__device__ inline bool touches_axis(int s, int origin, int teff, int span, int64_t nf)
+__device__ inline int rl(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ inline float rlf(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+
+// q = floor(n / d) for 0 <= n < 4096 and 1 <= d <= 1024 via a float reciprocal:
+// (n + 0.5)/d stays >= 0.5/d away from an integer, well above the rounding error.
+__device__ inline int div_rcp(int n, float rcp_d) { return (int)(((float)n + 0.5f) * rcp_d); }
+
+template <int D, int NS, int NCG>
+__global__ __launch_bounds__(64) void k_spread(SpreadArgs a)
 {
-    int v = s - origin;
-    if (v < 0) v += (int)nf;
-    return v < teff || v + span - 1 >= nf;
-}
+    extern __shared__ float2 lds_spread[];
+    const int lane = threadIdx.x;
+    const int4 item = a.items[blockIdx.x];
+    const int cg0 = blockIdx.y * NCG;
+    const int ncg = (int)min((int64_t)NCG, a.nc - cg0); // coils of this group that exist
+    const int TC = a.TC;
+    const int T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
+    const int T0 = a.tile[0];
 
-template <int D, int NS>
-__global__ __launch_bounds__(64 * kSpreadWaves) void k_spread(SpreadArgs a)
-{
-    constexpr int W = 2 * NS;
-    constexpr int NT = (D == 3) ? W * W * W : W * W;
-    extern __shared__ float4 lds_spread[];
+    for (int i = lane; i < NCG * TC; i += 64) lds_spread[i] = make_float2(0.f, 0.f);
 
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int64_t tile_id = (int64_t)blockIdx.x * kSpreadWaves + wave;
-    const int64_t coil = blockIdx.y;
-    if (tile_id >= a.K) return; // wave-uniform; no block barriers below
-
-    const int TC = a.tile[0] * a.tile[1] * (D == 3 ? a.tile[2] : 1);
-    float4 *acc = lds_spread + (int64_t)wave * TC;
-
-    // tile coordinates
-    int q[3], org[3], teff[3];
-    {
-        int64_t r = tile_id;
-        for (int ax = D - 1; ax >= 0; --ax) {
-            q[ax] = (int)(r % a.ntiles[ax]);
-            r /= a.ntiles[ax];
+    for (int base = item.y; base < item.z; base += 64) {
+        const int q = base + lane;
+        const bool valid = q < item.z;
+        int4 pa = make_int4(0, 0, 0, 0);
+        float4 pb = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) {
+            pa = a.pair_a[q];
+            const int4 b = a.pair_b[q];
+            pb = make_float4(__int_as_float(b.x), __int_as_float(b.y), __int_as_float(b.z), __int_as_float(b.w));
         }
-        for (int ax = 0; ax < D; ++ax) {
-            org[ax] = q[ax] * a.tile[ax];
-            teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
+        const float scale = a.use_dcf ? pb.w : 1.0f;
+        float vr[NCG], vi[NCG];
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) {
+            float2 v = make_float2(0.f, 0.f);
+            if (valid && c < ncg) {
+                const int64_t coil = cg0 + c;
+                v = (a.layout == GN_COIL_LAST) ? a.in[(int64_t)pa.x * a.nc + coil] : a.in[coil * a.M + pa.x];
+            }
+            vr[c] = v.x * scale;
+            vi[c] = v.y * scale;
+        }
+        const int n = min(64, item.z - base);
+        for (int k = 0; k < n; ++k) {
+            const int bj = rl(pa.y, k), bc = rl(pa.z, k), bl = rl(pa.w, k);
+            const int j0 = bj & 0xff, j1 = (bj >> 8) & 0xff, j2 = (bj >> 16) & 0xff;
+            const int c0 = bc & 0xff, c1 = (bc >> 8) & 0xff, c2 = (bc >> 16) & 0xff;
+            const int l0 = bl & 0x3ff, l1 = (bl >> 10) & 0x3ff, l2 = (bl >> 20) & 0x3ff;
+            const float f0 = rlf(pb.x, k) + (float)(NS - j0);
+            const float f1 = rlf(pb.y, k) + (float)(NS - j1);
+            const float f2 = rlf(pb.z, k) + (float)(NS - j2);
+            const int c12 = c1 * c2;
+            const int nbox = c0 * c12;
+            const float r12 = __builtin_amdgcn_rcpf((float)c12);
+            const float r2 = __builtin_amdgcn_rcpf((float)c2);
+            float svr[NCG], svi[NCG];
+#pragma unroll
+            for (int c = 0; c < NCG; ++c) {
+                svr[c] = rlf(vr[c], k);
+                svi[c] = rlf(vi[c], k);
+            }
+            for (int tap = lane; tap < nbox; tap += 64) {
+                const int i0 = div_rcp(tap, r12);
+                const int rem = tap - i0 * c12;
+                const int i1 = D == 3 ? div_rcp(rem, r2) : rem;
+                const int i2 = D == 3 ? rem - i1 * c2 : 0;
+                const float u0 = f0 - (float)i0;
+                const float u1 = f1 - (float)i1;
+                float w = exp2f(-a.coef[0] * u0 * u0) * exp2f(-a.coef[1] * u1 * u1);
+                int x0 = l0 + i0;
+                if (x0 >= T0) x0 -= T0; // only when a tile spans a whole axis
+                int x1 = l1 + i1;
+                if (x1 >= T1) x1 -= T1;
+                int off = x0 * T1 + x1;
+                if (D == 3) {
+                    const float u2 = f2 - (float)i2;
+                    w *= exp2f(-a.coef[2] * u2 * u2);
+                    int x2 = l2 + i2;
+                    if (x2 >= T2) x2 -= T2;
+                    off = off * T2 + x2;
+                }
+                float2 acc[NCG];
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) acc[c] = lds_spread[c * TC + off];
+#pragma unroll
+                for (int c = 0; c < NCG; ++c)
+                    lds_spread[c * TC + off] = make_float2(acc[c].x + svr[c] * w, acc[c].y + svi[c] * w);
+            }
         }
     }
 
-    for (int c = lane; c < TC; c += 64) acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-
-    const int np0 = a.npred[0], np1 = a.npred[1], np2 = (D == 3) ? a.npred[2] : 1;
-    for (int p0 = 0; p0 < np0; ++p0)
-        for (int p1 = 0; p1 < np1; ++p1)
-            for (int p2 = 0; p2 < np2; ++p2) {
-                int bq0 = q[0] - p0, bq1 = q[1] - p1;
-                if (bq0 < 0) bq0 += a.ntiles[0];
-                if (bq1 < 0) bq1 += a.ntiles[1];
-                int64_t bin = (int64_t)bq0 * a.ntiles[1] + bq1;
-                if (D == 3) {
-                    int bq2 = q[2] - p2;
-                    if (bq2 < 0) bq2 += a.ntiles[2];
-                    bin = bin * a.ntiles[2] + bq2;
-                }
-                const int bbeg = a.bin_start[bin];
-                const int bend = a.bin_start[bin + 1];
-                for (int base = bbeg; base < bend; base += 64) {
-                    const int j = base + lane;
-                    bool touch = false;
-                    if (j < bend) {
-                        const int4 rc = a.rec_cell[j];
-                        touch = touches_axis<D>(rc.x, org[0], teff[0], W, a.nf[0]) &&
-                                touches_axis<D>(rc.y, org[1], teff[1], W, a.nf[1]) &&
-                                (D == 2 || touches_axis<D>(rc.z, org[2], teff[2], W, a.nf[2]));
-                    }
-                    unsigned long long mask = __ballot(touch);
-                    while (mask) {
-                        const int k = __ffsll((long long)mask) - 1;
-                        mask &= mask - 1;
-                        const int idx = base + k;
-                        const int4 rc = a.rec_cell[idx];
-                        const float4 rf = a.rec_frac[idx];
-                        const int64_t orig = rc.w;
-                        float2 v = (a.layout == GN_COIL_LAST) ? a.in[orig * a.nc + coil] : a.in[coil * a.M + orig];
-                        if (a.use_dcf) {
-                            v.x *= rf.w;
-                            v.y *= rf.w;
-                        }
-                        for (int tap = lane; tap < NT; tap += 64) {
-                            int j0, j1, j2 = 0;
-                            if (D == 3) {
-                                j2 = tap % W;
-                                j1 = (tap / W) % W;
-                                j0 = tap / (W * W);
-                            } else {
-                                j1 = tap % W;
-                                j0 = tap / W;
-                            }
-                            int c0 = rc.x + j0;
-                            if (c0 >= a.nf[0]) c0 -= (int)a.nf[0];
-                            int l0 = c0 - org[0];
-                            int c1 = rc.y + j1;
-                            if (c1 >= a.nf[1]) c1 -= (int)a.nf[1];
-                            int l1 = c1 - org[1];
-                            bool in = (l0 >= 0) & (l0 < teff[0]) & (l1 >= 0) & (l1 < teff[1]);
-                            int l2 = 0;
-                            if (D == 3) {
-                                int c2 = rc.z + j2;
-                                if (c2 >= a.nf[2]) c2 -= (int)a.nf[2];
-                                l2 = c2 - org[2];
-                                in &= (l2 >= 0) & (l2 < teff[2]);
-                            }
-                            if (!in) continue;
-                            const float u0 = rf.x + (float)(NS - j0);
-                            const float u1 = rf.y + (float)(NS - j1);
-                            float w = exp2f(-a.coef[0] * u0 * u0) * exp2f(-a.coef[1] * u1 * u1);
-                            if (D == 3) {
-                                const float u2 = rf.z + (float)(NS - j2);
-                                w *= exp2f(-a.coef[2] * u2 * u2);
-                            }
-                            const int li = (D == 3) ? (l0 * a.tile[1] + l1) * a.tile[2] + l2 : l0 * a.tile[1] + l1;
-                            float4 s = acc[li];
-                            // Kahan: (s.x,s.y) running sum, (s.z,s.w) compensation
-                            const float yr = v.x * w - s.z;
-                            const float yi = v.y * w - s.w;
-                            const float tr = s.x + yr;
-                            const float ti = s.y + yi;
-                            s.z = (tr - s.x) - yr;
-                            s.w = (ti - s.y) - yi;
-                            s.x = tr;
-                            s.y = ti;
-                            acc[li] = s;
-                        }
-                    }
-                }
-            }
-
-    // write the tile once; consecutive lanes take consecutive cells of a row
-    float2 *g = a.grid + coil * a.G;
-    for (int c = lane; c < TC; c += 64) {
-        int l0, l1, l2 = 0;
-        if (D == 3) {
-            l2 = c % a.tile[2];
-            l1 = (c / a.tile[2]) % a.tile[1];
-            l0 = c / (a.tile[2] * a.tile[1]);
-        } else {
-            l1 = c % a.tile[1];
-            l0 = c / a.tile[1];
+    // write the tile once: to the grid, or to a partial slot when the tile was split
+    int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
+    {
+        int64_t r = item.x;
+        for (int ax = D - 1; ax >= 0; --ax) {
+            const int qq = (int)(r % a.ntiles[ax]);
+            r /= a.ntiles[ax];
+            org[ax] = qq * a.tile[ax];
+            teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
-        if (l0 >= teff[0] || l1 >= teff[1] || (D == 3 && l2 >= teff[2])) continue;
-        int64_t gi = (int64_t)(org[0] + l0) * a.nf[1] + (org[1] + l1);
-        if (D == 3) gi = gi * a.nf[2] + (org[2] + l2);
-        const float4 s = acc[c];
-        g[gi] = make_float2(s.x - s.z, s.y - s.w);
+    }
+    if (item.w >= 0) {
+        float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0) * TC;
+        for (int i = lane; i < ncg * TC; i += 64) dst[i] = lds_spread[i];
+        return;
+    }
+    for (int cidx = lane; cidx < TC; cidx += 64) {
+        int x0, x1, x2 = 0;
+        if (D == 3) {
+            x2 = cidx % T2;
+            x1 = (cidx / T2) % T1;
+            x0 = cidx / (T2 * T1);
+        } else {
+            x1 = cidx % T1;
+            x0 = cidx / T1;
+        }
+        if (x0 >= teff[0] || x1 >= teff[1] || (D == 3 && x2 >= teff[2])) continue;
+        int64_t gi = (int64_t)(org[0] + x0) * a.nf[1] + (org[1] + x1);
+        if (D == 3) gi = gi * a.nf[2] + (org[2] + x2);
+        for (int c = 0; c < ncg; ++c) a.grid[(int64_t)(cg0 + c) * a.G + gi] = lds_spread[c * TC + cidx];
+    }
+}
+
+// Kahan-compensated sum of the partial tiles of each split tile, in slice order.
+template <int D>
+__global__ __launch_bounds__(256) void k_spread_reduce(SpreadArgs a, const int4 *__restrict__ ritems)
+{
+    const int4 it = ritems[blockIdx.x]; // {tile, slot0, nslots}
+    int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
+    {
+        int64_t r = it.x;
+        for (int ax = D - 1; ax >= 0; --ax) {
+            const int q = (int)(r % a.ntiles[ax]);
+            r /= a.ntiles[ax];
+            org[ax] = q * a.tile[ax];
+            teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
+        }
+    }
+    const int TC = a.TC;
+    const int T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
+    for (int64_t e = threadIdx.x; e < (int64_t)TC * a.nc; e += blockDim.x) {
+        const int64_t coil = e / TC;
+        const int cidx = (int)(e % TC);
+        int x0, x1, x2 = 0;
+        if (D == 3) {
+            x2 = cidx % T2;
+            x1 = (cidx / T2) % T1;
+            x0 = cidx / (T2 * T1);
+        } else {
+            x1 = cidx % T1;
+            x0 = cidx / T1;
+        }
+        if (x0 >= teff[0] || x1 >= teff[1] || (D == 3 && x2 >= teff[2])) continue;
+        float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
+        for (int sl = 0; sl < it.z; ++sl) {
+            const float2 v = a.partial[((int64_t)(it.y + sl) * a.nc + coil) * TC + cidx];
+            const float yr = v.x - cr, yi = v.y - ci;
+            const float tr = sr + yr, ti = si + yi;
+            cr = (tr - sr) - yr;
+            ci = (ti - si) - yi;
+            sr = tr;
+            si = ti;
+        }
+        int64_t gi = (int64_t)(org[0] + x0) * a.nf[1] + (org[1] + x1);
+        if (D == 3) gi = gi * a.nf[2] + (org[2] + x2);
+        a.grid[coil * a.G + gi] = make_float2(sr - cr, si - ci);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Forward gather: one workgroup per home tile; the tile plus its window halo
-// is staged in LDS one coil at a time with coalesced loads, then every thread
-// interpolates one sample from LDS with separable weights in registers.
+// Forward gather: one workgroup per work item (a gather tile and up to
+// kGatherChunk of its samples).  The tile plus its window halo is staged in LDS
+// one coil at a time with coalesced loads; every thread then interpolates its
+// samples from LDS, forming the separable 1-D weights in registers.
 // ---------------------------------------------------------------------------
 
 struct GatherArgs {
@@ -435,149 +590,121 @@ struct GatherArgs {
     int tile[3];
     int ntiles[3];
     int region[3]; // staged cells per axis
-    int64_t K;
     int64_t G;
     int64_t M;
     int64_t nc;
     int layout;
     float coef[3];
-    const int *bin_start;
+    const int4 *items;
     const int4 *rec_cell;
     const float4 *rec_frac;
     const float2 *grid;
     float2 *out;
 };
 
-template <int D, int NS>
+template <int D, int NS, bool WRAP>
 __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
 {
     constexpr int W = 2 * NS;
     extern __shared__ float2 lds_gather[];
-    const int64_t tile_id = blockIdx.x;
-    const int bbeg = a.bin_start[tile_id];
-    const int bend = a.bin_start[tile_id + 1];
-    if (bbeg == bend) return;
+    const int4 item = a.items[blockIdx.x];
     const int cg0 = blockIdx.y * kCoilGroup;
     const int ncg = (int)min((int64_t)kCoilGroup, a.nc - cg0);
 
-    int q[3] = {0, 0, 0}, org[3] = {0, 0, 0};
+    int org[3] = {0, 0, 0};
     {
-        int64_t r = tile_id;
+        int64_t r = item.x;
         for (int ax = D - 1; ax >= 0; --ax) {
-            q[ax] = (int)(r % a.ntiles[ax]);
+            org[ax] = (int)(r % a.ntiles[ax]) * a.tile[ax];
             r /= a.ntiles[ax];
         }
-        for (int ax = 0; ax < D; ++ax) org[ax] = q[ax] * a.tile[ax];
     }
     const int R0 = a.region[0], R1 = a.region[1], R2 = (D == 3) ? a.region[2] : 1;
     const int RC = R0 * R1 * R2;
+    const int R12 = R1 * R2;
 
-    for (int cbase = bbeg; cbase < bend; cbase += kGatherThreads) {
-        const int j = cbase + threadIdx.x;
-        const bool valid = j < bend;
-        float w0[W], w1[W], w2[W];
-        int l0 = 0, l1 = 0, l2 = 0;
-        int64_t orig = 0;
-        if (valid) {
+    for (int cc = 0; cc < ncg; ++cc) {
+        const float2 *g = a.grid + (int64_t)(cg0 + cc) * a.G;
+        __syncthreads();
+        for (int rr = threadIdx.x; rr < RC; rr += kGatherThreads) {
+            const int r0 = rr / R12;
+            const int rem = rr - r0 * R12;
+            const int r1 = D == 3 ? rem / R2 : rem;
+            const int r2 = D == 3 ? rem - r1 * R2 : 0;
+            int g0 = org[0] + r0;
+            if (g0 >= a.nf[0]) g0 -= (int)a.nf[0];
+            int g1 = org[1] + r1;
+            if (g1 >= a.nf[1]) g1 -= (int)a.nf[1];
+            int64_t gi = (int64_t)g0 * a.nf[1] + g1;
+            if (D == 3) {
+                int g2 = org[2] + r2;
+                if (g2 >= a.nf[2]) g2 -= (int)a.nf[2];
+                gi = gi * a.nf[2] + g2;
+            }
+            lds_gather[rr] = g[gi];
+        }
+        __syncthreads();
+        for (int j = item.y + threadIdx.x; j < item.z; j += kGatherThreads) {
             const int4 rc = a.rec_cell[j];
             const float4 rf = a.rec_frac[j];
-            orig = rc.w;
-            l0 = rc.x - org[0];
+            int l0 = rc.x - org[0];
             if (l0 < 0) l0 += (int)a.nf[0];
-            l1 = rc.y - org[1];
+            int l1 = rc.y - org[1];
             if (l1 < 0) l1 += (int)a.nf[1];
+            int l2 = 0;
             if (D == 3) {
                 l2 = rc.z - org[2];
                 if (l2 < 0) l2 += (int)a.nf[2];
             }
+            float w1[W], w2[D == 3 ? W : 1];
 #pragma unroll
             for (int t = 0; t < W; ++t) {
-                const float u0 = rf.x + (float)(NS - t);
                 const float u1 = rf.y + (float)(NS - t);
-                w0[t] = exp2f(-a.coef[0] * u0 * u0);
                 w1[t] = exp2f(-a.coef[1] * u1 * u1);
                 if (D == 3) {
                     const float u2 = rf.z + (float)(NS - t);
                     w2[t] = exp2f(-a.coef[2] * u2 * u2);
                 }
             }
-        }
-        float2 acc[kCoilGroup];
+            float ar = 0.f, ai = 0.f;
+#pragma unroll 1
+            for (int t0 = 0; t0 < W; ++t0) {
+                const float u0 = rf.x + (float)(NS - t0);
+                const float w0 = exp2f(-a.coef[0] * u0 * u0);
+                int r0 = l0 + t0;
+                if (WRAP && r0 >= R0) r0 -= (int)a.nf[0]; // only when the region spans the whole axis
+                float pr = 0.f, pi = 0.f;
 #pragma unroll
-        for (int cc = 0; cc < kCoilGroup; ++cc) acc[cc] = make_float2(0.f, 0.f);
-
+                for (int t1 = 0; t1 < W; ++t1) {
+                    int r1 = l1 + t1;
+                    if (WRAP && r1 >= R1) r1 -= (int)a.nf[1];
+                    if (D == 3) {
+                        const int rowb = (r0 * R1 + r1) * R2;
+                        float qr = 0.f, qi = 0.f;
 #pragma unroll
-        for (int cc = 0; cc < kCoilGroup; ++cc) {
-            if (cc >= ncg) break;
-            const float2 *g = a.grid + (int64_t)(cg0 + cc) * a.G;
-            __syncthreads();
-            for (int r = threadIdx.x; r < RC; r += kGatherThreads) {
-                int r0, r1, r2 = 0;
-                if (D == 3) {
-                    r2 = r % R2;
-                    r1 = (r / R2) % R1;
-                    r0 = r / (R2 * R1);
-                } else {
-                    r1 = r % R1;
-                    r0 = r / R1;
-                }
-                int g0 = org[0] + r0;
-                if (g0 >= a.nf[0]) g0 -= (int)a.nf[0];
-                int g1 = org[1] + r1;
-                if (g1 >= a.nf[1]) g1 -= (int)a.nf[1];
-                int64_t gi = (int64_t)g0 * a.nf[1] + g1;
-                if (D == 3) {
-                    int g2 = org[2] + r2;
-                    if (g2 >= a.nf[2]) g2 -= (int)a.nf[2];
-                    gi = gi * a.nf[2] + g2;
-                }
-                lds_gather[r] = g[gi];
-            }
-            __syncthreads();
-            if (valid) {
-                float ar = 0.f, ai = 0.f;
-#pragma unroll
-                for (int t0 = 0; t0 < W; ++t0) {
-                    int r0 = l0 + t0;
-                    if (r0 >= R0) r0 -= (int)a.nf[0]; // only when the region spans the whole axis
-#pragma unroll
-                    for (int t1 = 0; t1 < W; ++t1) {
-                        int r1 = l1 + t1;
-                        if (r1 >= R1) r1 -= (int)a.nf[1];
-                        const float w01 = w0[t0] * w1[t1];
-                        if (D == 3) {
-                            const int rowb = (r0 * R1 + r1) * R2;
-                            float pr = 0.f, pi = 0.f;
-#pragma unroll
-                            for (int t2 = 0; t2 < W; ++t2) {
-                                int r2 = l2 + t2;
-                                if (r2 >= R2) r2 -= (int)a.nf[2];
-                                const float2 gv = lds_gather[rowb + r2];
-                                pr += w2[t2] * gv.x;
-                                pi += w2[t2] * gv.y;
-                            }
-                            ar += w01 * pr;
-                            ai += w01 * pi;
-                        } else {
-                            const float2 gv = lds_gather[r0 * R1 + r1];
-                            ar += w01 * gv.x;
-                            ai += w01 * gv.y;
+                        for (int t2 = 0; t2 < W; ++t2) {
+                            int r2 = l2 + t2;
+                            if (WRAP && r2 >= R2) r2 -= (int)a.nf[2];
+                            const float2 gv = lds_gather[rowb + r2];
+                            qr += w2[t2] * gv.x;
+                            qi += w2[t2] * gv.y;
                         }
+                        pr += w1[t1] * qr;
+                        pi += w1[t1] * qi;
+                    } else {
+                        const float2 gv = lds_gather[r0 * R1 + r1];
+                        pr += w1[t1] * gv.x;
+                        pi += w1[t1] * gv.y;
                     }
                 }
-                acc[cc] = make_float2(ar, ai);
+                ar += w0 * pr;
+                ai += w0 * pi;
             }
-        }
-        if (valid) {
-#pragma unroll
-            for (int cc = 0; cc < kCoilGroup; ++cc) {
-                if (cc >= ncg) break;
-                const int64_t coil = cg0 + cc;
-                if (a.layout == GN_COIL_LAST)
-                    a.out[orig * a.nc + coil] = acc[cc];
-                else
-                    a.out[coil * a.M + orig] = acc[cc];
-            }
+            const int64_t coil = cg0 + cc;
+            if (a.layout == GN_COIL_LAST)
+                a.out[(int64_t)rc.w * a.nc + coil] = make_float2(ar, ai);
+            else
+                a.out[coil * a.M + rc.w] = make_float2(ar, ai);
         }
     }
 }
@@ -692,7 +819,7 @@ template <typename T>
 int dev_alloc(T **p, size_t count, const char *what)
 {
     if (*p) {
-        hipFree(*p);
+        (void)hipFree(*p);
         *p = nullptr;
     }
     if (count == 0) count = 1;
@@ -704,26 +831,95 @@ int dev_alloc(T **p, size_t count, const char *what)
     return GN_OK;
 }
 
+// Scratch owner for one set_points call.
+struct Scratch {
+    std::vector<void *> ptrs;
+    template <typename T>
+    int alloc(T **p, size_t count, const char *what)
+    {
+        *p = nullptr;
+        int rc = dev_alloc(p, count, what);
+        if (!rc) ptrs.push_back((void *)*p);
+        return rc;
+    }
+    ~Scratch()
+    {
+        for (void *q : ptrs) (void)hipFree(q);
+    }
+};
+
+// Exclusive scan of n ints in place; writes the total to *total_dev (device int).
+hipError_t device_scan(int *data, int64_t n, int *bsum, hipStream_t st)
+{
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(kScanThreads), 0, st, data, n,
+                       bsum);
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kScanThreads), 0, st, bsum, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(kScanThreads), 0, st, data, n,
+                       bsum);
+    return hipGetLastError();
+}
+
+// Stable counting sort of n items by keys[i*stride] in [0,K).  Writes the sorted
+// item indices and the K+1 bucket offsets.
+int stable_sort_by_key(int64_t n, const int *keys, int stride, int64_t K, int *sorted, int *bin_start,
+                       hipStream_t st)
+{
+    Scratch sc;
+    int64_t C = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)(8 << 20) / std::max<int64_t>(K, 1)));
+    int64_t chunk_len = (n + C - 1) / C;
+    chunk_len = ((chunk_len + 63) / 64) * 64;
+    C = (n + chunk_len - 1) / chunk_len;
+    const int64_t nKC = K * C;
+    const int64_t nb = (nKC + kScanTile - 1) / kScanTile;
+    int *cnt = nullptr, *bsum = nullptr;
+    int rc;
+    if ((rc = sc.alloc(&cnt, (size_t)nKC, "sort counts")) || (rc = sc.alloc(&bsum, (size_t)nb + 1, "sort scan")))
+        return rc;
+    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(int) * nKC, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_bin_count, dim3((unsigned)C), dim3(256), 0, st, n, keys, stride, chunk_len, (int)C, cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = device_scan(cnt, nKC, bsum, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_bin_starts, dim3(grid_blocks(K + 1, 256, INT32_MAX)), dim3(256), 0, st, cnt, K, (int)C,
+                           n, bin_start);
+        hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)C), dim3(64), 0, st, n, keys, stride, chunk_len, (int)C, cnt,
+                           sorted);
+        e = hipGetLastError();
+    }
+    // keep the scratch alive until the stream has consumed it
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "stable sort");
+    return GN_OK;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
-// host launchers
+// host side
 // ---------------------------------------------------------------------------
 
-void default_tiles(int dim, int ns, const int64_t *nf, int *tile)
+void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
 {
     if (dim == 3) {
         tile[0] = 8;
         tile[1] = 8;
-        tile[2] = 16;
+        tile[2] = 8;
+        gtile[0] = 8;
+        gtile[1] = 8;
+        gtile[2] = 16;
     } else {
         tile[0] = 16;
         tile[1] = 32;
         tile[2] = 1;
+        gtile[0] = 32;
+        gtile[1] = 32;
+        gtile[2] = 1;
     }
-    for (int a = 0; a < dim; ++a)
-        if (tile[a] > nf[a]) tile[a] = (int)nf[a];
     (void)ns;
+    (void)nf;
 }
 
 #define GN_HIP(call)                                                                                                   \
@@ -736,122 +932,204 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
                  const float *dcf)
 {
     Geometry &g = p->geo;
+    Records &R = p->rec;
     hipStream_t st = p->stream;
+    const int64_t K = g.K, Kg = g.Kg;
     int rc;
-    if (M > p->rec.cap) {
-        if ((rc = dev_alloc(&p->rec.cell, (size_t)M, "records"))) return rc;
-        if ((rc = dev_alloc(&p->rec.frac, (size_t)M, "records"))) return rc;
-        p->rec.cap = M;
+    if (M > R.cap) {
+        if ((rc = dev_alloc(&R.cell, (size_t)M, "records")) || (rc = dev_alloc(&R.frac, (size_t)M, "records")))
+            return rc;
+        R.cap = M;
     }
-    if ((rc = dev_alloc(&p->rec.bin_start, (size_t)g.K + 1, "bin offsets"))) return rc;
-    if (M == 0) {
-        GN_HIP(hipMemsetAsync(p->rec.bin_start, 0, sizeof(int) * (g.K + 1), st));
-        GN_HIP(hipStreamSynchronize(st));
-        return GN_OK;
-    }
+    if ((rc = dev_alloc(&R.bin_start, (size_t)Kg + 1, "bin offsets"))) return rc;
+    R.npairs = 0;
 
-    // scratch: unsorted cell/frac, counts matrix, sorted index, scan block sums
-    const int64_t K = g.K;
-    int64_t C = std::max<int64_t>(1, std::min<int64_t>((M + 255) / 256, (int64_t)(8 << 20) / std::max<int64_t>(K, 1)));
-    int64_t chunk_len = (M + C - 1) / C;
-    chunk_len = ((chunk_len + 63) / 64) * 64;
-    C = (M + chunk_len - 1) / chunk_len;
-    const int64_t nKC = K * C;
-    const int64_t nb = (nKC + kScanTile - 1) / kScanTile;
+    Scratch sc;
+    std::vector<int> h_home(Kg + 1, 0), h_pair(K + 1, 0);
 
-    int4 *cell = nullptr;
-    float4 *frac = nullptr;
-    int *cnt = nullptr, *sorted = nullptr, *bsum = nullptr;
-    auto cleanup = [&]() {
-        hipFree(cell);
-        hipFree(frac);
-        hipFree(cnt);
-        hipFree(sorted);
-        hipFree(bsum);
-    };
-    if ((rc = dev_alloc(&cell, (size_t)M, "sort scratch")) || (rc = dev_alloc(&frac, (size_t)M, "sort scratch")) ||
-        (rc = dev_alloc(&cnt, (size_t)nKC, "sort scratch")) || (rc = dev_alloc(&sorted, (size_t)M, "sort scratch")) ||
-        (rc = dev_alloc(&bsum, (size_t)nb, "sort scratch"))) {
-        cleanup();
-        return rc;
-    }
-
-    hipError_t e = hipSuccess;
-    PrepArgs pa;
-    pa.M = M;
-    pa.x[0] = x;
-    pa.x[1] = y;
-    pa.x[2] = z;
-    pa.coord_f64 = coord_f64;
-    pa.df = df;
-    pa.dim = g.dim;
-    for (int a = 0; a < 3; ++a) {
-        pa.nf[a] = g.nf[a];
-        pa.tile[a] = g.tile[a];
-        pa.ntiles[a] = g.ntiles[a];
-    }
-    pa.ns = g.ns;
-    pa.cell = cell;
-    pa.frac = frac;
-    pa.err = p->err_flag;
-    if (e == hipSuccess) e = hipMemsetAsync(p->err_flag, 0, sizeof(int), st);
-    if (e == hipSuccess) {
+    if (M > 0) {
+        int4 *cell = nullptr;
+        float4 *frac = nullptr;
+        int *sorted = nullptr;
+        if ((rc = sc.alloc(&cell, (size_t)M, "sort scratch")) || (rc = sc.alloc(&frac, (size_t)M, "sort scratch")) ||
+            (rc = sc.alloc(&sorted, (size_t)M, "sort scratch")))
+            return rc;
+        PrepArgs pa;
+        pa.M = M;
+        pa.x[0] = x;
+        pa.x[1] = y;
+        pa.x[2] = z;
+        pa.coord_f64 = coord_f64;
+        pa.df = df;
+        pa.dim = g.dim;
+        for (int a = 0; a < 3; ++a) {
+            pa.nf[a] = g.nf[a];
+            pa.tile[a] = g.gtile[a];
+            pa.ntiles[a] = g.ngtiles[a];
+        }
+        pa.ns = g.ns;
+        pa.cell = cell;
+        pa.frac = frac;
+        pa.err = p->err_flag;
+        GN_HIP(hipMemsetAsync(p->err_flag, 0, sizeof(int), st));
         hipLaunchKernelGGL(k_prep, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, pa);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, sizeof(int) * nKC, st);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_bin_count, dim3((unsigned)C), dim3(256), 0, st, M, cell, chunk_len, (int)C, cnt);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, st, cnt, nKC, bsum);
-        hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kScanThreads), 0, st, bsum, nb);
-        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kScanThreads), 0, st, cnt, nKC, bsum);
-        hipLaunchKernelGGL(k_bin_starts, dim3(grid_blocks(K + 1, 256, INT32_MAX)), dim3(256), 0, st, cnt, K, (int)C,
-                           M, p->rec.bin_start);
-        hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)C), dim3(64), 0, st, M, cell, chunk_len, (int)C, cnt, sorted);
+        GN_HIP(hipGetLastError());
+        // gather-tile sort (stable) -> records in tile order
+        if ((rc = stable_sort_by_key(M, &cell[0].w, 4, Kg, sorted, R.bin_start, st))) return rc;
         hipLaunchKernelGGL(k_make_records, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, sorted, cell,
-                           frac, dcf, p->rec.cell, p->rec.frac);
-        e = hipGetLastError();
+                           frac, dcf, R.cell, R.frac);
+        GN_HIP(hipGetLastError());
+        int herr = 0;
+        GN_HIP(hipMemcpyAsync(&herr, p->err_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipStreamSynchronize(st));
+        if (herr) return fail(GN_EINVAL, "non-finite sample coordinate");
+
+        // (spread tile, sample) pair records, generated in record order
+        TileGeom tg;
+        tg.dim = g.dim;
+        for (int a = 0; a < 3; ++a) {
+            tg.nf[a] = g.nf[a];
+            tg.tile[a] = g.tile[a];
+            tg.ntiles[a] = g.ntiles[a];
+        }
+        tg.W = 2 * g.ns;
+        int *pcnt = nullptr, *pbsum = nullptr;
+        const int64_t nbp = (M + kScanTile - 1) / kScanTile;
+        if ((rc = sc.alloc(&pcnt, (size_t)M, "pair counts")) || (rc = sc.alloc(&pbsum, (size_t)nbp + 1, "pair scan")))
+            return rc;
+        hipLaunchKernelGGL(k_pair_count, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, R.cell, tg, pcnt);
+        int last_cnt = 0;
+        GN_HIP(hipMemcpyAsync(&last_cnt, pcnt + (M - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(device_scan(pcnt, M, pbsum, st));
+        int last_off = 0;
+        GN_HIP(hipMemcpyAsync(&last_off, pcnt + (M - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipStreamSynchronize(st));
+        const int64_t P = (int64_t)last_off + last_cnt;
+        if (P > (int64_t)INT32_MAX) return fail(GN_EINVAL, "too many (tile, sample) pairs");
+        int *pkey = nullptr, *psorted = nullptr, *pstart = nullptr;
+        int4 *ta = nullptr, *tb = nullptr;
+        if ((rc = sc.alloc(&pkey, (size_t)P, "pairs")) || (rc = sc.alloc(&ta, (size_t)P, "pairs")) ||
+            (rc = sc.alloc(&tb, (size_t)P, "pairs")) || (rc = sc.alloc(&psorted, (size_t)P, "pairs")) ||
+            (rc = sc.alloc(&pstart, (size_t)K + 1, "pairs")))
+            return rc;
+        hipLaunchKernelGGL(k_pair_emit, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, R.cell, R.frac, tg,
+                           pcnt, pkey, ta, tb);
+        GN_HIP(hipGetLastError());
+        if ((rc = stable_sort_by_key(P, pkey, 1, K, psorted, pstart, st))) return rc;
+        if (P > R.pairs_cap) {
+            if ((rc = dev_alloc(&R.pair_a, (size_t)P, "pairs")) || (rc = dev_alloc(&R.pair_b, (size_t)P, "pairs")))
+                return rc;
+            R.pairs_cap = P;
+        }
+        hipLaunchKernelGGL(k_permute_pairs, dim3(grid_blocks(P, 256, INT32_MAX)), dim3(256), 0, st, P, psorted, ta, tb,
+                           R.pair_a, R.pair_b);
+        GN_HIP(hipGetLastError());
+        GN_HIP(hipMemcpyAsync(h_home.data(), R.bin_start, sizeof(int) * (Kg + 1), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipMemcpyAsync(h_pair.data(), pstart, sizeof(int) * (K + 1), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipStreamSynchronize(st));
+        R.npairs = P;
+    } else {
+        GN_HIP(hipMemsetAsync(R.bin_start, 0, sizeof(int) * (Kg + 1), st));
+        GN_HIP(hipStreamSynchronize(st));
     }
-    int herr = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&herr, p->err_flag, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    cleanup();
-    if (e != hipSuccess) return hip_fail(e, "set_points");
-    if (herr) return fail(GN_EINVAL, "non-finite sample coordinate");
+
+    // work items: every spread tile gets at least one item (empty tiles write zeros)
+    std::vector<int4> sitems, ritems, gitems;
+    int slots = 0;
+    for (int64_t k = 0; k < K; ++k) {
+        const int b = h_pair[k], e = h_pair[k + 1];
+        const int n = e - b;
+        if (n <= kSpreadChunk) {
+            sitems.push_back(make_int4((int)k, b, e, -1));
+        } else {
+            const int nch = (n + kSpreadChunk - 1) / kSpreadChunk;
+            ritems.push_back(make_int4((int)k, slots, nch, 0));
+            for (int c = 0; c < nch; ++c)
+                sitems.push_back(make_int4((int)k, b + c * kSpreadChunk, std::min(e, b + (c + 1) * kSpreadChunk),
+                                           slots + c));
+            slots += nch;
+        }
+    }
+    for (int64_t k = 0; k < Kg; ++k) {
+        const int hb = h_home[k], he = h_home[k + 1];
+        for (int s0 = hb; s0 < he; s0 += kGatherChunk)
+            gitems.push_back(make_int4((int)k, s0, std::min(he, s0 + kGatherChunk), 0));
+    }
+    if ((rc = dev_alloc(&R.spread_items, sitems.size(), "work items")) ||
+        (rc = dev_alloc(&R.reduce_items, std::max<size_t>(1, ritems.size()), "work items")) ||
+        (rc = dev_alloc(&R.gather_items, std::max<size_t>(1, gitems.size()), "work items")))
+        return rc;
+    GN_HIP(hipMemcpy(R.spread_items, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice));
+    if (!ritems.empty())
+        GN_HIP(hipMemcpy(R.reduce_items, ritems.data(), sizeof(int4) * ritems.size(), hipMemcpyHostToDevice));
+    if (!gitems.empty())
+        GN_HIP(hipMemcpy(R.gather_items, gitems.data(), sizeof(int4) * gitems.size(), hipMemcpyHostToDevice));
+    R.n_spread_items = (int)sitems.size();
+    R.n_reduce_items = (int)ritems.size();
+    R.n_gather_items = (int)gitems.size();
+    R.n_slots = slots;
+    const int64_t TC = (int64_t)g.tile[0] * g.tile[1] * g.tile[2];
+    const int64_t need = (int64_t)slots * p->nc * TC;
+    if (need > R.partial_cap) {
+        if ((rc = dev_alloc(&R.partial, (size_t)need, "partial tiles"))) return rc;
+        R.partial_cap = need;
+    }
     return GN_OK;
 }
 
 namespace {
 
-template <int D, int NS>
-hipError_t spread_dispatch(const SpreadArgs &a, int64_t nc, hipStream_t st)
+// Raise the dynamic-LDS limit only for launches that need more than the default 64 KiB.
+template <typename K>
+void lds_limit(K kernel, size_t lds)
 {
-    const int TC = a.tile[0] * a.tile[1] * (D == 3 ? a.tile[2] : 1);
-    const size_t lds = (size_t)kSpreadWaves * TC * sizeof(float4);
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipFuncSetAttribute((const void *)k_spread<D, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
-    dim3 grid((unsigned)((a.K + kSpreadWaves - 1) / kSpreadWaves), (unsigned)nc);
-    hipLaunchKernelGGL((k_spread<D, NS>), grid, dim3(64 * kSpreadWaves), lds, st, a);
+    if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <int D, int NS, int NCG>
+hipError_t spread_launch(const SpreadArgs &a, int nitems, int64_t nc, hipStream_t st)
+{
+    const int ng = (int)((nc + NCG - 1) / NCG);
+    const size_t lds = (size_t)NCG * a.TC * sizeof(float2);
+    lds_limit(k_spread<D, NS, NCG>, lds);
+    hipLaunchKernelGGL((k_spread<D, NS, NCG>), dim3((unsigned)nitems, (unsigned)ng), dim3(64), lds, st, a);
     return hipGetLastError();
 }
 
 template <int D, int NS>
-hipError_t gather_dispatch(const GatherArgs &a, int64_t nc, hipStream_t st)
+hipError_t spread_dispatch(const SpreadArgs &a, int nitems, int nritems, const int4 *ritems, int64_t nc,
+                           hipStream_t st)
+{
+    hipError_t e;
+    if (nc >= 8)
+        e = spread_launch<D, NS, 8>(a, nitems, nc, st);
+    else if (nc >= 4)
+        e = spread_launch<D, NS, 4>(a, nitems, nc, st);
+    else if (nc >= 2)
+        e = spread_launch<D, NS, 2>(a, nitems, nc, st);
+    else
+        e = spread_launch<D, NS, 1>(a, nitems, nc, st);
+    if (e == hipSuccess && nritems > 0) {
+        hipLaunchKernelGGL(k_spread_reduce<D>, dim3((unsigned)nritems), dim3(256), 0, st, a, ritems);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+template <int D, int NS>
+hipError_t gather_dispatch(const GatherArgs &a, int nitems, int64_t nc, hipStream_t st)
 {
     const size_t lds = (size_t)a.region[0] * a.region[1] * (D == 3 ? a.region[2] : 1) * sizeof(float2);
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipFuncSetAttribute((const void *)k_gather<D, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
+    bool wrap = false;
+    for (int ax = 0; ax < D; ++ax) wrap |= a.region[ax] >= a.nf[ax];
+    dim3 grid((unsigned)nitems, (unsigned)((nc + kCoilGroup - 1) / kCoilGroup));
+    if (wrap) {
+        lds_limit(k_gather<D, NS, true>, lds);
+        hipLaunchKernelGGL((k_gather<D, NS, true>), grid, dim3(kGatherThreads), lds, st, a);
+    } else {
+        lds_limit(k_gather<D, NS, false>, lds);
+        hipLaunchKernelGGL((k_gather<D, NS, false>), grid, dim3(kGatherThreads), lds, st, a);
     }
-    dim3 grid((unsigned)a.K, (unsigned)((nc + kCoilGroup - 1) / kCoilGroup));
-    hipLaunchKernelGGL((k_gather<D, NS>), grid, dim3(kGatherThreads), lds, st, a);
     return hipGetLastError();
 }
 
@@ -880,24 +1158,25 @@ hipError_t gather_dispatch(const GatherArgs &a, int64_t nc, hipStream_t st)
     default: break;                                                                                                    \
     }
 
-hipError_t spread_any(int dim, int ns, const SpreadArgs &a, int64_t nc, hipStream_t st)
+hipError_t spread_any(int dim, int ns, const SpreadArgs &a, int nitems, int nritems, const int4 *ritems, int64_t nc,
+                      hipStream_t st)
 {
     if (dim == 3) {
-        GN_NS_CASES(3, spread_dispatch, a, nc, st)
+        GN_NS_CASES(3, spread_dispatch, a, nitems, nritems, ritems, nc, st)
     } else {
-        GN_NS_CASES(2, spread_dispatch, a, nc, st)
-        GN_NS_CASES_2D_EXTRA(spread_dispatch, a, nc, st)
+        GN_NS_CASES(2, spread_dispatch, a, nitems, nritems, ritems, nc, st)
+        GN_NS_CASES_2D_EXTRA(spread_dispatch, a, nitems, nritems, ritems, nc, st)
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t gather_any(int dim, int ns, const GatherArgs &a, int64_t nc, hipStream_t st)
+hipError_t gather_any(int dim, int ns, const GatherArgs &a, int nitems, int64_t nc, hipStream_t st)
 {
     if (dim == 3) {
-        GN_NS_CASES(3, gather_dispatch, a, nc, st)
+        GN_NS_CASES(3, gather_dispatch, a, nitems, nc, st)
     } else {
-        GN_NS_CASES(2, gather_dispatch, a, nc, st)
-        GN_NS_CASES_2D_EXTRA(gather_dispatch, a, nc, st)
+        GN_NS_CASES(2, gather_dispatch, a, nitems, nc, st)
+        GN_NS_CASES_2D_EXTRA(gather_dispatch, a, nitems, nc, st)
     }
     return hipErrorInvalidValue;
 }
@@ -932,26 +1211,22 @@ int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
         a.nf[ax] = g.nf[ax];
         a.tile[ax] = g.tile[ax];
         a.ntiles[ax] = g.ntiles[ax];
-        // A window starting in home tile q-p reaches tile q only if p <= 1 + (2ns-2)/T for
-        // every tile extent T it may jump over; the last tile of an axis can be short.
-        const int64_t last = g.nf[ax] - (int64_t)(g.ntiles[ax] - 1) * g.tile[ax];
-        const int tmin = (int)std::min<int64_t>(g.tile[ax], last);
-        const int np = 2 + (2 * g.ns - 2) / tmin;
-        a.npred[ax] = std::min(np, g.ntiles[ax]);
         a.coef[ax] = g.coef[ax];
     }
-    a.K = g.K;
     a.G = g.G;
     a.M = p->M;
     a.nc = p->nc;
+    a.TC = g.tile[0] * g.tile[1] * g.tile[2];
     a.layout = p->layout;
     a.use_dcf = use_dcf && p->has_dcf;
-    a.bin_start = p->rec.bin_start;
-    a.rec_cell = p->rec.cell;
-    a.rec_frac = p->rec.frac;
+    a.items = p->rec.spread_items;
+    a.pair_a = p->rec.pair_a;
+    a.pair_b = p->rec.pair_b;
     a.in = samples;
     a.grid = p->grid;
-    hipError_t e = spread_any(g.dim, g.ns, a, p->nc, p->stream);
+    a.partial = p->rec.partial;
+    hipError_t e = spread_any(g.dim, g.ns, a, p->rec.n_spread_items, p->rec.n_reduce_items, p->rec.reduce_items, p->nc,
+                              p->stream);
     if (e != hipSuccess) return hip_fail(e, "spread launch");
     return GN_OK;
 }
@@ -959,26 +1234,25 @@ int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
 int launch_gather(gn_plan_s *p, float2 *samples)
 {
     const Geometry &g = p->geo;
-    if (p->M == 0) return GN_OK;
+    if (p->M == 0 || p->rec.n_gather_items == 0) return GN_OK;
     GatherArgs a;
     for (int ax = 0; ax < 3; ++ax) {
         a.nf[ax] = g.nf[ax];
-        a.tile[ax] = g.tile[ax];
-        a.ntiles[ax] = g.ntiles[ax];
-        a.region[ax] = (int)std::min<int64_t>(g.tile[ax] + 2 * g.ns - 1, g.nf[ax]);
+        a.tile[ax] = g.gtile[ax];
+        a.ntiles[ax] = g.ngtiles[ax];
+        a.region[ax] = (int)std::min<int64_t>(g.gtile[ax] + 2 * g.ns - 1, g.nf[ax]);
         a.coef[ax] = g.coef[ax];
     }
-    a.K = g.K;
     a.G = g.G;
     a.M = p->M;
     a.nc = p->nc;
     a.layout = p->layout;
-    a.bin_start = p->rec.bin_start;
+    a.items = p->rec.gather_items;
     a.rec_cell = p->rec.cell;
     a.rec_frac = p->rec.frac;
     a.grid = p->grid;
     a.out = samples;
-    hipError_t e = gather_any(g.dim, g.ns, a, p->nc, p->stream);
+    hipError_t e = gather_any(g.dim, g.ns, a, p->rec.n_gather_items, p->nc, p->stream);
     if (e != hipSuccess) return hip_fail(e, "gather launch");
     return GN_OK;
 }

@@ -41,7 +41,7 @@ class Plan:
     """One NUFFT plan (C handle) bound to a device and its current stream."""
 
     def __init__(self, modes, ncoils=1, eps=None, rat=None, nspread=None, iflag=1, layout="coil_last",
-                 device=None, max_nspread=None, tiles=None):
+                 device=None, max_nspread=None, tiles=None, gather_tiles=None):
         L = _lib.load()
         self.modes = tuple(int(m) for m in modes)
         self.dim = len(self.modes)
@@ -66,8 +66,8 @@ class Plan:
             stream = torch.cuda.current_stream(self.device).cuda_stream
             check(L.gn_plan_create(self.dim, m, self.rat, self.nspread, self.iflag, self.ncoils, self.layout,
                                    ctypes.c_void_p(stream), ctypes.byref(self._h)))
-        if tiles is not None:
-            self.set_tiles(tiles)
+        if tiles is not None or gather_tiles is not None:
+            self.set_tiles(tiles, gather_tiles)
         nf = (ctypes.c_int64 * 3)()
         ns = ctypes.c_int()
         tau = ctypes.c_double()
@@ -78,15 +78,27 @@ class Plan:
         self._dcf = None
 
     # ------------------------------------------------------------------ setup
-    def set_tiles(self, tiles):
-        t = (ctypes.c_int * 3)(*(list(tiles) + [0] * (3 - len(tiles))))
+    def set_tiles(self, tiles=None, gather_tiles=None):
+        """Spread (output) and gather (home) tile shapes in cells; None/0 keeps the default.
+        Requires a new set_points afterwards."""
+        sp = list(tiles or []) + [0] * (3 - len(tiles or []))
+        ga = list(gather_tiles or []) + [0] * (3 - len(gather_tiles or []))
+        t = (ctypes.c_int * 6)(*(sp + ga))
         check(_lib.load().gn_plan_set_tiles(self._h, t))
+        self.M = None
+
+    def _tile_info(self):
+        t = (ctypes.c_int * 6)()
+        check(_lib.load().gn_plan_info(self._h, None, None, None, None, None, t))
+        return tuple(t[:self.dim]), tuple(t[3:3 + self.dim])
 
     @property
     def tiles(self):
-        t = (ctypes.c_int * 3)()
-        check(_lib.load().gn_plan_info(self._h, None, None, None, None, None, t))
-        return tuple(t[:self.dim])
+        return self._tile_info()[0]
+
+    @property
+    def gather_tiles(self):
+        return self._tile_info()[1]
 
     def _bind_stream(self):
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -171,6 +183,17 @@ class Plan:
         """Run on preallocated contiguous complex64 device tensors (no allocation, no checks of shape)."""
         check(_lib.load().gn_execute(self._h, kind, src.data_ptr(), dst.data_ptr()))
         return dst
+
+    def set_timing(self, enable=True):
+        """Enable per-stage HIP event timers inside gn_execute."""
+        check(_lib.load().gn_plan_set_timing(self._h, 1 if enable else 0))
+
+    def stage_times(self, reset=True):
+        """{stage: (total ms, launches)} accumulated since the last reset (synchronizes)."""
+        ms = (ctypes.c_double * 6)()
+        n = (ctypes.c_int64 * 6)()
+        check(_lib.load().gn_plan_stage_times(self._h, ms, n, 1 if reset else 0))
+        return {name: (ms[i], n[i]) for i, name in enumerate(_lib.STAGES)}
 
     def __del__(self):
         h = getattr(self, "_h", None)

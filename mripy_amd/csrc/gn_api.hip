@@ -76,9 +76,10 @@ static int setup_tiles(gn_plan_s *p)
     }
     // a window must not leave a tile and re-enter it around the periodic axis:
     // otherwise one tile spans the whole axis
-    auto fit = [&](int t, int a) {
-        t = (int)std::min<int64_t>(std::max(t, 1), g.nf[a]);
-        if (t + 2 * g.ns - 1 > g.nf[a]) t = (int)g.nf[a];
+    // spread tiles are at most 32 cells per axis (32-bit cell masks in the kernel)
+    auto fit = [&](int t, int a, int tmax) {
+        t = (int)std::min<int64_t>(std::min(std::max(t, 1), tmax), g.nf[a]);
+        if (t + 2 * g.ns - 1 > g.nf[a]) t = g.nf[a] <= tmax ? (int)g.nf[a] : (int)(g.nf[a] - 2 * g.ns + 1);
         return t;
     };
     g.K = 1;
@@ -89,8 +90,8 @@ static int setup_tiles(gn_plan_s *p)
             g.ntiles[a] = g.ngtiles[a] = 1;
             continue;
         }
-        g.tile[a] = fit(tile[a], a);
-        g.gtile[a] = fit(gtile[a], a);
+        g.tile[a] = fit(tile[a], a, 32);
+        g.gtile[a] = fit(gtile[a], a, 1 << 20);
         g.ntiles[a] = (int)((g.nf[a] + g.tile[a] - 1) / g.tile[a]);
         g.ngtiles[a] = (int)((g.nf[a] + g.gtile[a] - 1) / g.gtile[a]);
         g.K *= g.ntiles[a];
@@ -102,10 +103,11 @@ static int setup_tiles(gn_plan_s *p)
     const int64_t ncg = std::min<int64_t>(p->nc, kCoilGroup);
     int64_t rc = 1;
     for (int a = 0; a < g.dim; ++a) rc *= std::min<int64_t>(g.gtile[a] + 2 * g.ns - 1, g.nf[a]);
-    if (tc * ncg * 8 > 160 * 1024 || rc * 8 > 160 * 1024)
+    const int64_t stage = 64 * (int64_t)(g.tile[0] + g.tile[1] + g.tile[2]) * 4 + 64 * ncg * 8 + 64 * 48;
+    (void)tc;
+    if (stage > 160 * 1024 || rc * 8 > 160 * 1024)
         return fail(GN_EINVAL, "tile shape exceeds the 160 KiB LDS budget");
-    for (int a = 0; a < g.dim; ++a)
-        if (g.tile[a] > 1024) return fail(GN_EINVAL, "spread tile extent above 1024 cells");
+
     if (g.K > (int64_t)INT32_MAX / 2 || g.Kg > (int64_t)INT32_MAX / 2) return fail(GN_EINVAL, "too many tiles");
     return GN_OK;
 }

@@ -372,30 +372,37 @@ __global__ __launch_bounds__(256) void k_permute_pairs(int64_t n, const int *__r
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint spread: output-stationary LDS tiles.
+// Adjoint spread: output-stationary tiles accumulated in registers.
 //
-// One wave owns one work item: a tile and a slice of at most kSpreadChunk of
-// the (tile, sample) pair records whose windows touch it.  The 64 records of a
-// batch are loaded with coalesced loads (one per lane, with the sample's coil
-// values) and broadcast one at a time with readlane; the lanes then take the
-// taps of the window that fall inside the tile.  Each tap's weight is formed
-// once and accumulated for up to 8 coils by a read-modify-write of per-coil
-// LDS planes.  Lanes of one instruction touch distinct cells, a tile slice
-// belongs to one wave and LDS operations of a wave complete in program order,
-// so no atomics are needed and the summation order is fixed: the result is
-// deterministic.  The tile is written to HBM once, coalesced; slices of a tile
-// that was split (dense k-space centre) go to partial buffers that
-// k_spread_reduce combines with Kahan compensation in slice order.
+// A work item is a tile and a slice of at most kSpreadChunk of the (tile,
+// sample) pair records whose windows touch it.  The workgroup splits the tile
+// into blocks of 64 cells (4x4x4 in 3-D, 8x8 in 2-D), one block per wave and
+// one cell per lane; every lane keeps its cell's value for up to 8 coils in
+// registers.  Pair records are staged in LDS 64 at a time: the sample's coil
+// values (times dcf) and, per axis, a table of the separable kernel weight at
+// every tile-local cell, zero outside the window.  A wave then visits the
+// staged pairs whose windows reach its block; each lane forms its weight from
+// three table reads and accumulates all coils with FMAs.  Nothing is written
+// to shared memory by more than one wave, there are no atomics, and every
+// lane adds its contributions in the fixed pair order: the result is
+// deterministic.  After each batch the plain FP32 batch sums (at most 64
+// terms) are folded into Kahan-compensated (sum, compensation) registers.
+// The tile is then written to HBM once, coalesced through LDS; slices of a
+// split tile go to partial buffers combined by k_spread_reduce.
 // ---------------------------------------------------------------------------
 
 struct SpreadArgs {
     int64_t nf[3];
     int tile[3];
     int ntiles[3];
+    int blk[3];  // block (cells per wave) per axis
+    int nblk[3]; // blocks per tile per axis
     int64_t G;
     int64_t M;
     int64_t nc;
     int TC;
+    int TT;      // tile[0] + tile[1] + tile[2]: weight-table entries per pair
+    int ns;
     int layout;
     int use_dcf;
     float coef[3];
@@ -407,99 +414,197 @@ struct SpreadArgs {
     float2 *partial;
 };
 
-__device__ inline int rl(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
-__device__ inline float rlf(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+constexpr int kSpreadBatch = 64;
+constexpr int kSpreadWavesMax = 8;
 
-// q = floor(n / d) for 0 <= n < 4096 and 1 <= d <= 1024 via a float reciprocal:
-// (n + 0.5)/d stays >= 0.5/d away from an integer, well above the rounding error.
-__device__ inline int div_rcp(int n, float rcp_d) { return (int)(((float)n + 0.5f) * rcp_d); }
-
-template <int D, int NS, int NCG>
-__global__ __launch_bounds__(64) void k_spread(SpreadArgs a)
+template <int D, int NCG>
+__global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
 {
-    extern __shared__ float2 lds_spread[];
-    const int lane = threadIdx.x;
+    extern __shared__ float lds_sp[];
+    // LDS: weight tables [batch][TT] | coil values [batch][NCG] float2 | masks [batch][3]
+    float *s_tab = lds_sp;
+    float2 *s_val = (float2 *)(lds_sp + kSpreadBatch * a.TT);
+    int *s_msk = (int *)(s_val + kSpreadBatch * NCG);
+    int4 *s_pa = (int4 *)(s_msk + kSpreadBatch * 4);
+    float4 *s_pb = (float4 *)(s_pa + kSpreadBatch);
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int nwaves = blockDim.x >> 6;
     const int4 item = a.items[blockIdx.x];
     const int cg0 = blockIdx.y * NCG;
-    const int ncg = (int)min((int64_t)NCG, a.nc - cg0); // coils of this group that exist
-    const int TC = a.TC;
-    const int T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
-    const int T0 = a.tile[0];
+    const int ncg = (int)min((int64_t)NCG, a.nc - cg0);
+    const int T0 = a.tile[0], T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
 
-    for (int i = lane; i < NCG * TC; i += 64) lds_spread[i] = make_float2(0.f, 0.f);
-
-    for (int base = item.y; base < item.z; base += 64) {
-        const int q = base + lane;
-        const bool valid = q < item.z;
-        int4 pa = make_int4(0, 0, 0, 0);
-        float4 pb = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (valid) {
-            pa = a.pair_a[q];
-            const int4 b = a.pair_b[q];
-            pb = make_float4(__int_as_float(b.x), __int_as_float(b.y), __int_as_float(b.z), __int_as_float(b.w));
+    // this wave's block (block index = blockIdx.z * waves + wave) and this lane's cell
+    const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
+    const int bidx = blockIdx.z * nwaves + wave;
+    const bool has_block = bidx < nb;
+    int bq0, bq1, bq2 = 0;
+    if (D == 3) {
+        bq2 = bidx % a.nblk[2];
+        bq1 = (bidx / a.nblk[2]) % a.nblk[1];
+        bq0 = bidx / (a.nblk[2] * a.nblk[1]);
+    } else {
+        bq1 = bidx % a.nblk[1];
+        bq0 = bidx / a.nblk[1];
+    }
+    int L0, L1, L2 = 0;
+    if (D == 3) {
+        L2 = bq2 * a.blk[2] + (lane % a.blk[2]);
+        L1 = bq1 * a.blk[1] + (lane / a.blk[2]) % a.blk[1];
+        L0 = bq0 * a.blk[0] + lane / (a.blk[2] * a.blk[1]);
+    } else {
+        L1 = bq1 * a.blk[1] + (lane % a.blk[1]);
+        L0 = bq0 * a.blk[0] + lane / a.blk[1];
+    }
+    const bool in_tile = has_block && L0 < T0 && L1 < T1 && L2 < T2;
+    // lanes of a partial block read a valid table entry; their result is never stored
+    L0 = min(L0, T0 - 1);
+    L1 = min(L1, T1 - 1);
+    L2 = min(L2, T2 - 1);
+    // bit masks of the block's cells per axis (tile extents <= 32)
+    uint32_t bm0 = 0, bm1 = 0, bm2 = 1;
+    if (has_block) {
+        for (int i = 0; i < a.blk[0]; ++i) bm0 |= (bq0 * a.blk[0] + i < T0) ? (1u << (bq0 * a.blk[0] + i)) : 0u;
+        for (int i = 0; i < a.blk[1]; ++i) bm1 |= (bq1 * a.blk[1] + i < T1) ? (1u << (bq1 * a.blk[1] + i)) : 0u;
+        if (D == 3) {
+            bm2 = 0;
+            for (int i = 0; i < a.blk[2]; ++i) bm2 |= (bq2 * a.blk[2] + i < T2) ? (1u << (bq2 * a.blk[2] + i)) : 0u;
         }
-        const float scale = a.use_dcf ? pb.w : 1.0f;
-        float vr[NCG], vi[NCG];
+    }
+    const int to1 = T0, to2 = T0 + T1; // table offsets of axes 1 and 2
+
+    float sr[NCG], si[NCG], kr[NCG], ki[NCG];
 #pragma unroll
-        for (int c = 0; c < NCG; ++c) {
-            float2 v = make_float2(0.f, 0.f);
-            if (valid && c < ncg) {
-                const int64_t coil = cg0 + c;
-                v = (a.layout == GN_COIL_LAST) ? a.in[(int64_t)pa.x * a.nc + coil] : a.in[coil * a.M + pa.x];
-            }
-            vr[c] = v.x * scale;
-            vi[c] = v.y * scale;
-        }
-        const int n = min(64, item.z - base);
-        for (int k = 0; k < n; ++k) {
-            const int bj = rl(pa.y, k), bc = rl(pa.z, k), bl = rl(pa.w, k);
-            const int j0 = bj & 0xff, j1 = (bj >> 8) & 0xff, j2 = (bj >> 16) & 0xff;
-            const int c0 = bc & 0xff, c1 = (bc >> 8) & 0xff, c2 = (bc >> 16) & 0xff;
-            const int l0 = bl & 0x3ff, l1 = (bl >> 10) & 0x3ff, l2 = (bl >> 20) & 0x3ff;
-            const float f0 = rlf(pb.x, k) + (float)(NS - j0);
-            const float f1 = rlf(pb.y, k) + (float)(NS - j1);
-            const float f2 = rlf(pb.z, k) + (float)(NS - j2);
-            const int c12 = c1 * c2;
-            const int nbox = c0 * c12;
-            const float r12 = __builtin_amdgcn_rcpf((float)c12);
-            const float r2 = __builtin_amdgcn_rcpf((float)c2);
-            float svr[NCG], svi[NCG];
+    for (int c = 0; c < NCG; ++c) sr[c] = si[c] = kr[c] = ki[c] = 0.f;
+
+    for (int base = item.y; base < item.z; base += kSpreadBatch) {
+        const int n = min(kSpreadBatch, item.z - base);
+        __syncthreads(); // previous batch fully consumed
+        // phase 1: one thread per pair loads its record and coil values, builds the masks
+        if (tid < n) {
+            const int q = base + tid;
+            const int4 pa = a.pair_a[q];
+            const int4 pbi = a.pair_b[q];
+            const float4 pb = make_float4(__int_as_float(pbi.x), __int_as_float(pbi.y), __int_as_float(pbi.z),
+                                          __int_as_float(pbi.w));
+            s_pa[tid] = pa;
+            s_pb[tid] = pb;
+            const float scale = a.use_dcf ? pb.w : 1.0f;
 #pragma unroll
             for (int c = 0; c < NCG; ++c) {
-                svr[c] = rlf(vr[c], k);
-                svi[c] = rlf(vi[c], k);
-            }
-            for (int tap = lane; tap < nbox; tap += 64) {
-                const int i0 = div_rcp(tap, r12);
-                const int rem = tap - i0 * c12;
-                const int i1 = D == 3 ? div_rcp(rem, r2) : rem;
-                const int i2 = D == 3 ? rem - i1 * c2 : 0;
-                const float u0 = f0 - (float)i0;
-                const float u1 = f1 - (float)i1;
-                float w = exp2f(-a.coef[0] * u0 * u0) * exp2f(-a.coef[1] * u1 * u1);
-                int x0 = l0 + i0;
-                if (x0 >= T0) x0 -= T0; // only when a tile spans a whole axis
-                int x1 = l1 + i1;
-                if (x1 >= T1) x1 -= T1;
-                int off = x0 * T1 + x1;
-                if (D == 3) {
-                    const float u2 = f2 - (float)i2;
-                    w *= exp2f(-a.coef[2] * u2 * u2);
-                    int x2 = l2 + i2;
-                    if (x2 >= T2) x2 -= T2;
-                    off = off * T2 + x2;
+                float2 v = make_float2(0.f, 0.f);
+                if (c < ncg) {
+                    const int64_t coil = cg0 + c;
+                    v = (a.layout == GN_COIL_LAST) ? a.in[(int64_t)pa.x * a.nc + coil] : a.in[coil * a.M + pa.x];
                 }
-                float2 acc[NCG];
-#pragma unroll
-                for (int c = 0; c < NCG; ++c) acc[c] = lds_spread[c * TC + off];
-#pragma unroll
-                for (int c = 0; c < NCG; ++c)
-                    lds_spread[c * TC + off] = make_float2(acc[c].x + svr[c] * w, acc[c].y + svi[c] * w);
+                s_val[tid * NCG + c] = make_float2(v.x * scale, v.y * scale);
             }
+            // valid tile-local cells of the window per axis: (L - l) mod T < c
+            const int Ts[3] = {T0, T1, T2};
+            for (int ax = 0; ax < D; ++ax) {
+                const int l = (ax == 0 ? pa.w : ax == 1 ? (pa.w >> 10) : (pa.w >> 20)) & 0x3ff;
+                const int c = (pa.z >> (8 * ax)) & 0xff;
+                uint32_t m = 0;
+                for (int r = 0; r < c; ++r) {
+                    int L = l + r;
+                    if (L >= Ts[ax]) L -= Ts[ax];
+                    m |= 1u << L;
+                }
+                s_msk[tid * 4 + ax] = (int)m;
+            }
+        }
+        __syncthreads();
+        // phase 2: weight tables, one entry per (pair, axis, tile-local cell)
+        for (int e = tid; e < n * a.TT; e += blockDim.x) {
+            const int k = e / a.TT;
+            const int i = e - k * a.TT;
+            const int ax = i < to1 ? 0 : (i < to2 ? 1 : 2);
+            const int L = i - (ax == 0 ? 0 : (ax == 1 ? to1 : to2));
+            const int4 pa = s_pa[k];
+            const float4 pb = s_pb[k];
+            const int Tax = ax == 0 ? T0 : (ax == 1 ? T1 : T2);
+            const int l = (ax == 0 ? pa.w : ax == 1 ? (pa.w >> 10) : (pa.w >> 20)) & 0x3ff;
+            const int c = (pa.z >> (8 * ax)) & 0xff;
+            const int j = (pa.y >> (8 * ax)) & 0xff;
+            int rel = L - l;
+            if (rel < 0) rel += Tax;
+            float w = 0.f;
+            if (rel < c) {
+                const float t = ax == 0 ? pb.x : (ax == 1 ? pb.y : pb.z);
+                const float u = t + (float)(a.ns - j - rel);
+                w = exp2f(-a.coef[ax] * u * u);
+            }
+            s_tab[e] = w;
+        }
+        __syncthreads();
+        // phase 3: each wave visits the pairs that reach its block
+        bool hit = false;
+        if (lane < n && has_block) {
+            const int m0 = s_msk[lane * 4], m1 = s_msk[lane * 4 + 1];
+            const int m2 = D == 3 ? s_msk[lane * 4 + 2] : 1;
+            hit = ((uint32_t)m0 & bm0) && ((uint32_t)m1 & bm1) && ((uint32_t)m2 & bm2);
+        }
+        unsigned long long todo = __ballot(hit);
+        float ar[NCG], ai[NCG];
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) ar[c] = ai[c] = 0.f;
+        // visits in groups of up to 4 so their LDS reads are in flight together; a
+        // missing visit reuses pair 0 with weight 0 (adds exact zeros)
+        while (todo) {
+            int kk[4];
+            float wz[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool some = todo != 0;
+                kk[u] = some ? __ffsll((long long)todo) - 1 : 0;
+                wz[u] = some ? 1.f : 0.f;
+                todo &= todo - 1;
+            }
+            float w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float *tab = s_tab + kk[u] * a.TT;
+                float ww = tab[L0] * tab[to1 + L1];
+                if (D == 3) ww *= tab[to2 + L2];
+                w[u] = ww * wz[u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float2 *v = s_val + kk[u] * NCG;
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) {
+                    const float2 vc = v[c];
+                    ar[c] = fmaf(w[u], vc.x, ar[c]);
+                    ai[c] = fmaf(w[u], vc.y, ai[c]);
+                }
+            }
+        }
+        // Kahan fold of this batch's partial sums
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) {
+            const float yr = ar[c] - kr[c], yi = ai[c] - ki[c];
+            const float tr = sr[c] + yr, ti = si[c] + yi;
+            kr[c] = (tr - sr[c]) - yr;
+            ki[c] = (ti - si[c]) - yi;
+            sr[c] = tr;
+            si[c] = ti;
         }
     }
 
-    // write the tile once: to the grid, or to a partial slot when the tile was split
+    // write the tile once, straight from registers
+    if (!in_tile) return;
+    const int TC = a.TC;
+    const int cidx = (L0 * T1 + L1) * T2 + L2;
+    if (item.w >= 0) {
+        float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0) * TC + cidx;
+#pragma unroll
+        for (int c = 0; c < NCG; ++c)
+            if (c < ncg) dst[(int64_t)c * TC] = make_float2(sr[c] - kr[c], si[c] - ki[c]);
+        return;
+    }
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     {
         int64_t r = item.x;
@@ -510,26 +615,12 @@ __global__ __launch_bounds__(64) void k_spread(SpreadArgs a)
             teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
     }
-    if (item.w >= 0) {
-        float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0) * TC;
-        for (int i = lane; i < ncg * TC; i += 64) dst[i] = lds_spread[i];
-        return;
-    }
-    for (int cidx = lane; cidx < TC; cidx += 64) {
-        int x0, x1, x2 = 0;
-        if (D == 3) {
-            x2 = cidx % T2;
-            x1 = (cidx / T2) % T1;
-            x0 = cidx / (T2 * T1);
-        } else {
-            x1 = cidx % T1;
-            x0 = cidx / T1;
-        }
-        if (x0 >= teff[0] || x1 >= teff[1] || (D == 3 && x2 >= teff[2])) continue;
-        int64_t gi = (int64_t)(org[0] + x0) * a.nf[1] + (org[1] + x1);
-        if (D == 3) gi = gi * a.nf[2] + (org[2] + x2);
-        for (int c = 0; c < ncg; ++c) a.grid[(int64_t)(cg0 + c) * a.G + gi] = lds_spread[c * TC + cidx];
-    }
+    if (L0 >= teff[0] || L1 >= teff[1] || (D == 3 && L2 >= teff[2])) return;
+    int64_t gi = (int64_t)(org[0] + L0) * a.nf[1] + (org[1] + L1);
+    if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
+#pragma unroll
+    for (int c = 0; c < NCG; ++c)
+        if (c < ncg) a.grid[(int64_t)(cg0 + c) * a.G + gi] = make_float2(sr[c] - kr[c], si[c] - ki[c]);
 }
 
 // Kahan-compensated sum of the partial tiles of each split tile, in slice order.
@@ -904,8 +995,8 @@ int stable_sort_by_key(int64_t n, const int *keys, int stride, int64_t K, int *s
 void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
 {
     if (dim == 3) {
-        tile[0] = 8;
-        tile[1] = 8;
+        tile[0] = 4;
+        tile[1] = 4;
         tile[2] = 8;
         gtile[0] = 8;
         gtile[1] = 8;
@@ -1086,29 +1177,39 @@ void lds_limit(K kernel, size_t lds)
     if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-template <int D, int NS, int NCG>
+inline size_t spread_lds_bytes(const SpreadArgs &a, int ncg)
+{
+    return (size_t)kSpreadBatch * a.TT * 4 + (size_t)kSpreadBatch * ncg * 8 + (size_t)kSpreadBatch * 16 +
+           (size_t)kSpreadBatch * 32;
+}
+
+template <int D, int NCG>
 hipError_t spread_launch(const SpreadArgs &a, int nitems, int64_t nc, hipStream_t st)
 {
     const int ng = (int)((nc + NCG - 1) / NCG);
-    const size_t lds = (size_t)NCG * a.TC * sizeof(float2);
-    lds_limit(k_spread<D, NS, NCG>, lds);
-    hipLaunchKernelGGL((k_spread<D, NS, NCG>), dim3((unsigned)nitems, (unsigned)ng), dim3(64), lds, st, a);
+    const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
+    const int waves = std::min(nb, kSpreadWavesMax);
+    const int nbg = (nb + waves - 1) / waves;
+    const size_t lds = spread_lds_bytes(a, NCG);
+    lds_limit(k_spread<D, NCG>, lds);
+    hipLaunchKernelGGL((k_spread<D, NCG>), dim3((unsigned)nitems, (unsigned)ng, (unsigned)nbg), dim3(64 * waves), lds,
+                       st, a);
     return hipGetLastError();
 }
 
-template <int D, int NS>
+template <int D>
 hipError_t spread_dispatch(const SpreadArgs &a, int nitems, int nritems, const int4 *ritems, int64_t nc,
                            hipStream_t st)
 {
     hipError_t e;
     if (nc >= 8)
-        e = spread_launch<D, NS, 8>(a, nitems, nc, st);
+        e = spread_launch<D, 8>(a, nitems, nc, st);
     else if (nc >= 4)
-        e = spread_launch<D, NS, 4>(a, nitems, nc, st);
+        e = spread_launch<D, 4>(a, nitems, nc, st);
     else if (nc >= 2)
-        e = spread_launch<D, NS, 2>(a, nitems, nc, st);
+        e = spread_launch<D, 2>(a, nitems, nc, st);
     else
-        e = spread_launch<D, NS, 1>(a, nitems, nc, st);
+        e = spread_launch<D, 1>(a, nitems, nc, st);
     if (e == hipSuccess && nritems > 0) {
         hipLaunchKernelGGL(k_spread_reduce<D>, dim3((unsigned)nritems), dim3(256), 0, st, a, ritems);
         e = hipGetLastError();
@@ -1161,13 +1262,9 @@ hipError_t gather_dispatch(const GatherArgs &a, int nitems, int64_t nc, hipStrea
 hipError_t spread_any(int dim, int ns, const SpreadArgs &a, int nitems, int nritems, const int4 *ritems, int64_t nc,
                       hipStream_t st)
 {
-    if (dim == 3) {
-        GN_NS_CASES(3, spread_dispatch, a, nitems, nritems, ritems, nc, st)
-    } else {
-        GN_NS_CASES(2, spread_dispatch, a, nitems, nritems, ritems, nc, st)
-        GN_NS_CASES_2D_EXTRA(spread_dispatch, a, nitems, nritems, ritems, nc, st)
-    }
-    return hipErrorInvalidValue;
+    (void)ns;
+    if (dim == 3) return spread_dispatch<3>(a, nitems, nritems, ritems, nc, st);
+    return spread_dispatch<2>(a, nitems, nritems, ritems, nc, st);
 }
 
 hipError_t gather_any(int dim, int ns, const GatherArgs &a, int nitems, int64_t nc, hipStream_t st)
@@ -1213,10 +1310,16 @@ int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
         a.ntiles[ax] = g.ntiles[ax];
         a.coef[ax] = g.coef[ax];
     }
+    for (int ax = 0; ax < 3; ++ax) {
+        a.blk[ax] = ax < g.dim ? (g.dim == 3 ? 4 : 8) : 1;
+        a.nblk[ax] = (g.tile[ax] + a.blk[ax] - 1) / a.blk[ax];
+    }
     a.G = g.G;
     a.M = p->M;
     a.nc = p->nc;
     a.TC = g.tile[0] * g.tile[1] * g.tile[2];
+    a.TT = g.tile[0] + g.tile[1] + (g.dim == 3 ? g.tile[2] : 0);
+    a.ns = g.ns;
     a.layout = p->layout;
     a.use_dcf = use_dcf && p->has_dcf;
     a.items = p->rec.spread_items;

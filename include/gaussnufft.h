@@ -119,6 +119,11 @@ int gn_plan_set_tiles(gn_plan *plan, const int *tile);
 int gn_plan_set_timing(gn_plan *plan, int enable);
 int gn_plan_stage_times(gn_plan *plan, double *ms, int64_t *counts, int reset);
 
+/* Copy the sample order chosen by gn_set_points (for each sorted position the
+ * original sample index; M ints) to host memory.  No counterpart in the
+ * reference: it exposes the stable counting-sort result to tests and tools. */
+int gn_plan_sample_order(const gn_plan *plan, int *order_host);
+
 int gn_plan_destroy(gn_plan *plan);
 
 #ifdef __cplusplus

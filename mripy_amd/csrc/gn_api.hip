@@ -465,6 +465,18 @@ int gn_plan_stage_times(gn_plan *p, double *ms, int64_t *counts, int reset)
     return GN_OK;
 }
 
+int gn_plan_sample_order(const gn_plan *p, int *order)
+{
+    g_err.clear();
+    if (!p || !order) return fail(GN_EINVAL, "bad arguments");
+    if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called first");
+    if (p->M == 0) return GN_OK;
+    hipError_t e = hipMemcpy2D(order, sizeof(int), (const char *)p->rec.cell + 3 * sizeof(int), sizeof(int4),
+                               sizeof(int), (size_t)p->M, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "sample order copy");
+    return GN_OK;
+}
+
 int gn_plan_destroy(gn_plan *p)
 {
     if (!p) return GN_OK;

@@ -184,6 +184,13 @@ class Plan:
         check(_lib.load().gn_execute(self._h, kind, src.data_ptr(), dst.data_ptr()))
         return dst
 
+    def sample_order(self):
+        """Original sample index of every sorted position (the counting-sort result)."""
+        out = np.zeros(self.M or 0, np.int32)
+        if self.M:
+            check(_lib.load().gn_plan_sample_order(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
+
     def set_timing(self, enable=True):
         """Enable per-stage HIP event timers inside gn_execute."""
         check(_lib.load().gn_plan_set_timing(self._h, 1 if enable else 0))

@@ -94,6 +94,15 @@ int gn_set_points(gn_plan *plan, int64_t M, const void *x, const void *y, const 
  * stream.  in/out: device complex64 buffers in the plan's layout.  Asynchronous. */
 int gn_execute(gn_plan *plan, int kind, const void *in, void *out);
 
+/* Gradient of the forward map with respect to the sample coordinates
+ * (no counterpart in the reference; for autograd): for image f and complex
+ * per-sample weights w (device, sample layout of the plan),
+ *   grad[a*M + j] = sum_c Re( conj(w[j,c]) * d(GN_TYPE2 f)[j,c] / dx_a[j] ),
+ * from the analytic derivative of the Gaussian kernel, in the original sample
+ * order (float32 device buffer of dim*M).  With w = dL/dy this is dL/dx for
+ * y = A(x) f; with f = dL/d(image) and w = y it is dL/dx for image = A(x)^H y. */
+int gn_traj_grad(gn_plan *plan, const void *image, const void *weights, float *grad);
+
 /* Query: nf[3] oversampled grid, nspread, tau, number of samples, coils, and
  * tile[6] = spread (output) tile shape followed by the gather (home) tile shape. */
 int gn_plan_info(const gn_plan *plan, int64_t *nf, int *nspread, double *tau, int64_t *M, int64_t *ncoils,

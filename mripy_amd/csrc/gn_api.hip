@@ -346,6 +346,7 @@ int gn_set_points(gn_plan *p, int64_t M, const void *x, const void *y, const voi
     int rc = bin_and_sort(p, M, x, y, z, coord_f64, df, dcf);
     if (rc) return rc;
     p->M = M;
+    p->df = df;
     p->has_dcf = dcf != nullptr;
     p->has_points = true;
     return GN_OK;
@@ -407,6 +408,22 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
         break;
     }
     }
+    return rc;
+}
+
+int gn_traj_grad(gn_plan *p, const void *image, const void *weights, float *grad)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called before gn_traj_grad");
+    if (p->M == 0) return GN_OK;
+    if (!image || !weights || !grad) return fail(GN_EINVAL, "null data pointer");
+    const Geometry &g = p->geo;
+    const double pi = 3.141592653589793;
+    const double c = std::pow(pi / p->tau, g.dim / 2.0);
+    int rc = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)image, (float)(c / (double)g.G)); });
+    if (!rc) rc = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, p->iflag >= 0); });
+    if (!rc) rc = launch_traj_grad(p, (const float2 *)weights, grad);
     return rc;
 }
 

@@ -76,6 +76,7 @@ struct gn_plan_s {
     int gtile_override[3] = {0, 0, 0};
 
     int64_t M = 0;
+    double df = 1.0;
     bool has_points = false;
     bool has_dcf = false;
     gn::Records rec;
@@ -117,6 +118,7 @@ int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf);
 int launch_gather(gn_plan_s *p, float2 *samples);
 int launch_pad(gn_plan_s *p, const float2 *image, float scale);
 int launch_crop(gn_plan_s *p, float2 *image, float scale);
+int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out);
 void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile);
 
 } // namespace gn

@@ -801,6 +801,136 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Trajectory gradient: derivative of the type-2 interpolation with respect to
+// the sample coordinates, contracted with per-sample complex weights:
+//   out[a][j] = sum_c Re( conj(wts[j,c]) * d/dx_a (A image)[j,c] ),
+// with d/dx_a w = w * gscale_a * (t + ns - tap), gscale_a = -h_a df / (2 tau),
+// the analytic derivative of the Gaussian exp(-(xi - h g)^2 / (4 tau)).
+// Same staging as the gather; all coils of a sample are handled by one thread.
+// ---------------------------------------------------------------------------
+
+template <int D, int NS, bool WRAP>
+__global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, const float2 *__restrict__ wts,
+                                                               float *__restrict__ out, float gs0, float gs1, float gs2)
+{
+    constexpr int W = 2 * NS;
+    extern __shared__ float2 lds_gather[];
+    const int4 item = a.items[blockIdx.x];
+    int org[3] = {0, 0, 0};
+    {
+        int64_t r = item.x;
+        for (int ax = D - 1; ax >= 0; --ax) {
+            org[ax] = (int)(r % a.ntiles[ax]) * a.tile[ax];
+            r /= a.ntiles[ax];
+        }
+    }
+    const int R0 = a.region[0], R1 = a.region[1], R2 = (D == 3) ? a.region[2] : 1;
+    const int RC = R0 * R1 * R2;
+    const int R12 = R1 * R2;
+    for (int jb = item.y; jb < item.z; jb += kGatherThreads) {
+        const int j = jb + threadIdx.x;
+        const bool valid = j < item.z;
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+        int4 rc = make_int4(0, 0, 0, 0);
+        float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
+        int l0 = 0, l1 = 0, l2 = 0;
+        if (valid) {
+            rc = a.rec_cell[j];
+            rf = a.rec_frac[j];
+            l0 = rc.x - org[0];
+            if (l0 < 0) l0 += (int)a.nf[0];
+            l1 = rc.y - org[1];
+            if (l1 < 0) l1 += (int)a.nf[1];
+            if (D == 3) {
+                l2 = rc.z - org[2];
+                if (l2 < 0) l2 += (int)a.nf[2];
+            }
+        }
+        for (int64_t coil = 0; coil < a.nc; ++coil) {
+            const float2 *g = a.grid + coil * a.G;
+            __syncthreads();
+            for (int rr = threadIdx.x; rr < RC; rr += kGatherThreads) {
+                const int r0 = rr / R12;
+                const int rem = rr - r0 * R12;
+                const int r1 = D == 3 ? rem / R2 : rem;
+                const int r2 = D == 3 ? rem - r1 * R2 : 0;
+                int q0 = org[0] + r0;
+                if (q0 >= a.nf[0]) q0 -= (int)a.nf[0];
+                int q1 = org[1] + r1;
+                if (q1 >= a.nf[1]) q1 -= (int)a.nf[1];
+                int64_t gi = (int64_t)q0 * a.nf[1] + q1;
+                if (D == 3) {
+                    int q2 = org[2] + r2;
+                    if (q2 >= a.nf[2]) q2 -= (int)a.nf[2];
+                    gi = gi * a.nf[2] + q2;
+                }
+                lds_gather[rr] = g[gi];
+            }
+            __syncthreads();
+            if (!valid) continue;
+            // value sums with plain and derivative weights, separably
+            float dxr = 0.f, dxi = 0.f, dyr = 0.f, dyi = 0.f, dzr = 0.f, dzi = 0.f;
+#pragma unroll 1
+            for (int t0 = 0; t0 < W; ++t0) {
+                const float u0 = rf.x + (float)(NS - t0);
+                const float w0 = exp2f(-a.coef[0] * u0 * u0);
+                int r0 = l0 + t0;
+                if (WRAP && r0 >= R0) r0 -= (int)a.nf[0];
+                float pr = 0.f, pi = 0.f, pyr = 0.f, pyi = 0.f, pzr = 0.f, pzi = 0.f;
+                for (int t1 = 0; t1 < W; ++t1) {
+                    const float u1 = rf.y + (float)(NS - t1);
+                    const float w1 = exp2f(-a.coef[1] * u1 * u1);
+                    int r1 = l1 + t1;
+                    if (WRAP && r1 >= R1) r1 -= (int)a.nf[1];
+                    float qr = 0.f, qi = 0.f, qzr = 0.f, qzi = 0.f;
+                    if (D == 3) {
+                        const int rowb = (r0 * R1 + r1) * R2;
+#pragma unroll
+                        for (int t2 = 0; t2 < W; ++t2) {
+                            const float u2 = rf.z + (float)(NS - t2);
+                            const float w2 = exp2f(-a.coef[2] * u2 * u2);
+                            int r2 = l2 + t2;
+                            if (WRAP && r2 >= R2) r2 -= (int)a.nf[2];
+                            const float2 gv = lds_gather[rowb + r2];
+                            qr += w2 * gv.x;
+                            qi += w2 * gv.y;
+                            qzr += w2 * u2 * gv.x;
+                            qzi += w2 * u2 * gv.y;
+                        }
+                    } else {
+                        const float2 gv = lds_gather[r0 * R1 + r1];
+                        qr = gv.x;
+                        qi = gv.y;
+                    }
+                    pr += w1 * qr;
+                    pi += w1 * qi;
+                    pyr += w1 * u1 * qr;
+                    pyi += w1 * u1 * qi;
+                    pzr += w1 * qzr;
+                    pzi += w1 * qzi;
+                }
+                dxr += w0 * u0 * pr;
+                dxi += w0 * u0 * pi;
+                dyr += w0 * pyr;
+                dyi += w0 * pyi;
+                dzr += w0 * pzr;
+                dzi += w0 * pzi;
+            }
+            const float2 wv = (a.layout == GN_COIL_LAST) ? wts[(int64_t)rc.w * a.nc + coil] : wts[coil * a.M + rc.w];
+            // Re(conj(wv) * d) = wv.x d.x + wv.y d.y
+            g0 += wv.x * dxr + wv.y * dxi;
+            g1 += wv.x * dyr + wv.y * dyi;
+            g2 += wv.x * dzr + wv.y * dzi;
+        }
+        if (valid) {
+            out[rc.w] = g0 * gs0;
+            out[a.M + rc.w] = g1 * gs1;
+            if (D == 3) out[2 * a.M + rc.w] = g2 * gs2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Fused pre-pass (deapodize + zero-pad + coil transpose) and post-pass
 // (crop + deapodize + scale + coil transpose).  Frequency k = n - m//2 sits at
 // grid index k mod nf (fft/nufft_func.py:761-764, 728-729).
@@ -1357,6 +1487,68 @@ int launch_gather(gn_plan_s *p, float2 *samples)
     a.out = samples;
     hipError_t e = gather_any(g.dim, g.ns, a, p->rec.n_gather_items, p->nc, p->stream);
     if (e != hipSuccess) return hip_fail(e, "gather launch");
+    return GN_OK;
+}
+
+namespace {
+template <int D, int NS>
+hipError_t traj_grad_dispatch(const GatherArgs &a, int nitems, const float2 *wts, float *out, const float *gs,
+                              hipStream_t st)
+{
+    const size_t lds = (size_t)a.region[0] * a.region[1] * (D == 3 ? a.region[2] : 1) * sizeof(float2);
+    bool wrap = false;
+    for (int ax = 0; ax < D; ++ax) wrap |= a.region[ax] >= a.nf[ax];
+    if (wrap) {
+        lds_limit(k_traj_grad<D, NS, true>, lds);
+        hipLaunchKernelGGL((k_traj_grad<D, NS, true>), dim3((unsigned)nitems), dim3(kGatherThreads), lds, st, a, wts,
+                           out, gs[0], gs[1], gs[2]);
+    } else {
+        lds_limit(k_traj_grad<D, NS, false>, lds);
+        hipLaunchKernelGGL((k_traj_grad<D, NS, false>), dim3((unsigned)nitems), dim3(kGatherThreads), lds, st, a,
+                           wts, out, gs[0], gs[1], gs[2]);
+    }
+    return hipGetLastError();
+}
+
+hipError_t traj_grad_any(int dim, int ns, const GatherArgs &a, int nitems, const float2 *wts, float *out,
+                         const float *gs, hipStream_t st)
+{
+    if (dim == 3) {
+        GN_NS_CASES(3, traj_grad_dispatch, a, nitems, wts, out, gs, st)
+    } else {
+        GN_NS_CASES(2, traj_grad_dispatch, a, nitems, wts, out, gs, st)
+        GN_NS_CASES_2D_EXTRA(traj_grad_dispatch, a, nitems, wts, out, gs, st)
+    }
+    return hipErrorInvalidValue;
+}
+} // namespace
+
+int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out)
+{
+    const Geometry &g = p->geo;
+    if (p->M == 0 || p->rec.n_gather_items == 0) return GN_OK;
+    GatherArgs a;
+    for (int ax = 0; ax < 3; ++ax) {
+        a.nf[ax] = g.nf[ax];
+        a.tile[ax] = g.gtile[ax];
+        a.ntiles[ax] = g.ngtiles[ax];
+        a.region[ax] = (int)std::min<int64_t>(g.gtile[ax] + 2 * g.ns - 1, g.nf[ax]);
+        a.coef[ax] = g.coef[ax];
+    }
+    a.G = g.G;
+    a.M = p->M;
+    a.nc = p->nc;
+    a.layout = p->layout;
+    a.items = p->rec.gather_items;
+    a.rec_cell = p->rec.cell;
+    a.rec_frac = p->rec.frac;
+    a.grid = p->grid;
+    a.out = nullptr;
+    float gs[3] = {0.f, 0.f, 0.f};
+    const double pi = 3.141592653589793;
+    for (int ax = 0; ax < g.dim; ++ax) gs[ax] = (float)(-(2 * pi / (double)g.nf[ax]) * p->df / (2.0 * p->tau));
+    hipError_t e = traj_grad_any(g.dim, g.ns, a, p->rec.n_gather_items, wts, out, gs, p->stream);
+    if (e != hipSuccess) return hip_fail(e, "trajectory gradient launch");
     return GN_OK;
 }
 

@@ -131,6 +131,7 @@ class Plan:
                                             float(df), d.data_ptr() if d is not None else None))
         self.M = M
         self._dcf = d
+        self._points = [c for c in coords]
         return self
 
     # ------------------------------------------------------------------ shapes
@@ -183,6 +184,20 @@ class Plan:
         """Run on preallocated contiguous complex64 device tensors (no allocation, no checks of shape)."""
         check(_lib.load().gn_execute(self._h, kind, src.data_ptr(), dst.data_ptr()))
         return dst
+
+    def traj_grad(self, image, weights):
+        """(dim, M) float32: sum_c Re(conj(w) d(type2 image)/dx) per sample and axis (analytic)."""
+        if self.M is None:
+            raise RuntimeError("set_points must be called first")
+        img = torch.as_tensor(image).to(device=self.device, dtype=torch.complex64).reshape(self.image_shape())
+        w = torch.as_tensor(weights).to(device=self.device, dtype=torch.complex64).reshape(self.sample_shape())
+        img = img.contiguous()
+        w = w.contiguous()
+        out = torch.zeros((self.dim, self.M), dtype=torch.float32, device=self.device)
+        with torch.cuda.device(self.device):
+            self._bind_stream()
+            check(_lib.load().gn_traj_grad(self._h, img.data_ptr(), w.data_ptr(), out.data_ptr()))
+        return out
 
     def sample_order(self):
         """Original sample index of every sorted position (the counting-sort result)."""

@@ -323,6 +323,44 @@ def adjoint_exact(coords, y, p: Params, fast=False):
     return type1(coords, y, p, fast) * M
 
 
+def traj_grad(coords, Fk, w, p: Params):
+    """FP64 analytic coordinate gradient of the type-2 map (no reference counterpart).
+
+    out[a, j] = sum_c Re(conj(w[j,c]) * d y[j,c] / d x_a[j]) with y = type2(Fk) and
+    d/dxi_a exp(-(xi_a - h g_a)^2/(4 tau)) = -(xi_a - h g_a)/(2 tau) * exp(...),
+    times df (xi = df x mod 2 pi).
+    """
+    coords = [np.asarray(x, dtype=np.float64) * p.df for x in coords]
+    M = coords[0].shape[0]
+    F2, _ = _as_coil_last(Fk, p.modes)
+    grid = place(F2 * deapod(p.modes, p.tau)[..., None], p.nf)
+    axes = tuple(range(p.d))
+    grid = np.fft.ifftn(grid, axes=axes) * p.G if p.iflag < 0 else np.fft.fftn(grid, axes=axes)
+    g2 = grid.reshape(p.G, -1)
+    w2 = np.asarray(w).reshape(M, -1)
+    idx, wt, dw = [], [], []
+    for a, x in enumerate(coords):
+        xi, m = cell_index(x, p.nf[a])
+        h = TWO_PI / p.nf[a]
+        g = m[:, None] + np.arange(-p.ns, p.ns)[None, :]
+        d = xi[:, None] - h * g
+        e = np.exp(-0.25 * d ** 2 / p.tau)
+        idx.append(np.mod(g, p.nf[a]))
+        wt.append(e)
+        dw.append(-d / (2 * p.tau) * e)
+    out = np.zeros((p.d, M))
+    for a in range(p.d):
+        flat = np.zeros((M, 1), dtype=np.int64)
+        ww = np.ones((M, 1))
+        for b in range(p.d):
+            f = dw[b] if b == a else wt[b]
+            flat = (flat[:, :, None] * p.nf[b] + idx[b][:, None, :]).reshape(M, -1)
+            ww = (ww[:, :, None] * f[:, None, :]).reshape(M, -1)
+        dy = np.einsum('mt,mtc->mc', ww, g2[flat]) / p.G
+        out[a] = np.sum(np.real(np.conj(w2) * dy), axis=1) * p.df
+    return out
+
+
 # ----------------------------------------------------------------------------
 # reference function names (fft/nufft_func.py:702,741,780,1167,1209,1261)
 # ----------------------------------------------------------------------------

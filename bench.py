@@ -23,6 +23,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3d_ute_8coil.txt")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(stage, tiles, nspread):
+    """HBM bytes per launch of ``stage`` from the committed FETCH_SIZE/WRITE_SIZE
+    counter summary (lines ``traffic <stage> <bytes> tiles <t0,t1,t2> nspread <n>``),
+    or None when no measurement exists for this kernel configuration."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.split()
+                if (len(w) == 7 and w[0] == "traffic" and w[1] == stage and w[4] == ",".join(map(str, tiles))
+                        and int(w[6]) == nspread):
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
 
 
 def algorithmic_bytes(d, N, G, M, nc):
@@ -37,10 +59,11 @@ def algorithmic_bytes(d, N, G, M, nc):
     }
 
 
-def cpu_baseline(cfg, modes, nc, fraction):
-    """Time the FP64 C oracle (restated reference loop nests) on a bounded sample:
-    every ``fraction``-th spoke, full grids and FFTs; the gridding time is scaled
-    back to the full sample count."""
+def cpu_baseline(cfg, modes, nc, fraction, reps=3):
+    """Time the FP64 C oracle (the reference loop nests restated) on the same workload:
+    every ``fraction``-th spoke (1 = all), full grids and FFTs; one warm-up pass, then
+    the median of ``reps`` passes.  The gridding time is scaled back to the full
+    sample count when a fraction is used."""
     import numpy as np
 
     from oracle import coracle as C
@@ -54,32 +77,39 @@ def cpu_baseline(cfg, modes, nc, fraction):
     img = cfg["image"].astype(np.complex128)
     threads = min(nc, cores)   # numba prange over coils (fft/nufft_func.py:935,1036)
     C.lib()
-    t0 = time.perf_counter()
-    F2 = img * O.deapod(p.modes, p.tau)[..., None]
-    grid = O.place(F2, p.nf)
-    grid = C._fftn(grid, (0, 1, 2), False, max(1, cores - 1))
-    t1 = time.perf_counter()
-    y = C.gather(coords, grid, p.nf, p.ns, p.tau, threads)
-    t2 = time.perf_counter()
-    g1 = C.spread(coords, y, p.nf, p.ns, p.tau, threads)
-    t3 = time.perf_counter()
-    F = C._fftn(g1, (0, 1, 2), True, max(1, cores - 1))
-    out = O.crop(F, p.modes) * O.deapod(p.modes, p.tau)[..., None] / Ms
-    t4 = time.perf_counter()
-    t_fixed = (t1 - t0) + (t4 - t3)
-    t_grid = (t2 - t1) + (t3 - t2)
+
+    def one():
+        t0 = time.perf_counter()
+        F2 = img * O.deapod(p.modes, p.tau)[..., None]
+        grid = O.place(F2, p.nf)
+        grid = C._fftn(grid, (0, 1, 2), False, max(1, cores - 1))
+        t1 = time.perf_counter()
+        y = C.gather(coords, grid, p.nf, p.ns, p.tau, threads)
+        t2 = time.perf_counter()
+        g1 = C.spread(coords, y, p.nf, p.ns, p.tau, threads)
+        t3 = time.perf_counter()
+        F = C._fftn(g1, (0, 1, 2), True, max(1, cores - 1))
+        out = O.crop(F, p.modes) * O.deapod(p.modes, p.tau)[..., None] / Ms
+        t4 = time.perf_counter()
+        del out
+        return (t1 - t0) + (t4 - t3), (t3 - t1)
+
+    one()
+    runs = sorted((one() for _ in range(reps)), key=lambda r: r[0] + r[1])
+    t_fixed, t_grid = runs[len(runs) // 2]
     M_full = cfg["coords"][0].shape[0]
     t_full = t_fixed + t_grid * (M_full / Ms)
-    del out
+    what = "all %d samples" % M_full if fraction == 1 else "%d of %d samples (every %dth spoke) scaled x%d" % (
+        Ms, M_full, fraction, fraction)
     return {
         "value": M_full * nc / t_full,
         "unit": "coil-samples/s",
         "cores": cores,
         "kind": "port",
-        "sample": ("FP64 C restatement of the reference loop nests (oracle/gridding.c), gridding on "
-                   "%d of %d samples (every %dth spoke, %d threads over coils) scaled x%d, plus full 256^3x%d "
-                   "FFTs (scipy, %d workers) and pad/crop; measured %.2fs grid + %.2fs fft/pre/post"
-                   % (Ms, M_full, fraction, threads, fraction, nc, max(1, cores - 1), t_grid, t_fixed)),
+        "sample": ("one fwd+adj of the full workload by the FP64 C restatement of the reference loop nests "
+                   "(oracle/gridding.c): gridding on %s with %d threads over coils, 256^3x%d FFTs (scipy, "
+                   "%d workers), pad/crop; median of %d after 1 warm-up: %.2fs gridding + %.2fs fft/pre/post"
+                   % (what, threads, nc, max(1, cores - 1), reps, t_grid, t_fixed)),
     }
 
 
@@ -89,9 +119,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-fraction", type=int, default=4)
+    ap.add_argument("--cpu-fraction", type=int, default=1)
     ap.add_argument("--nspread", type=int, default=3)
-    ap.add_argument("--tiles", type=str, default=None, help="spread/gather tile shape, e.g. 8,8,16")
+    ap.add_argument("--tiles", type=str, default=None, help="spread tile shape, e.g. 4,4,8")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one plain 1 GiB read + 1 GiB write streaming kernel first (HBM counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -108,6 +140,7 @@ def main():
     from mripy_amd.plan import Plan
 
     modes, nc, nspokes, nread = (128, 128, 128), 8, 4096, 256
+    log("generating the 3-D UTE workload")
     cfg = data.config_3d_ute(n=128, nc=nc, nspokes=nspokes, nread=nread)
     d, N, G = 3, int(np.prod(modes)), 256 ** 3
     M = cfg["coords"][0].shape[0]
@@ -128,6 +161,17 @@ def main():
         P.execute_into(GN_TYPE2, img, ybuf)
         P.execute_into(GN_TYPE1, ybuf, obuf)
 
+    if args.copy_probe:
+        # reads exactly COPY_BYTES and writes exactly COPY_BYTES: the reference point for
+        # scaling FETCH_SIZE / WRITE_SIZE counter readings to bytes
+        a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
+    log("set_points done; warm-up")
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -145,6 +189,7 @@ def main():
     wall = time.perf_counter() - t0
     st = P.stage_times(reset=True)
     P.set_timing(False)
+    log("timed %d steps: %.3f ms/step" % (args.steps, ev0.elapsed_time(ev1) / args.steps))
     ms_step = ev0.elapsed_time(ev1) / args.steps
 
     ab = algorithmic_bytes(d, N, G, M, nc)
@@ -178,12 +223,14 @@ def main():
         "path_roofline": {"algorithmic_bytes": path_bytes, "achieved_GBps": path_bytes / (ms_step * 1e-3) / 1e9,
                           "frac": path_bytes / (ms_step * 1e-3) / HBM_PEAK},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(dom, list(P.tiles), args.nspread),
                      "bytes_per_launch": ab[dom], "avg_launch_ms": stages[dom]["ms"]},
         "stages": stages,
         "wall_s_timed": wall,
     }
     if not args.no_cpu:
+        log("CPU baseline")
         try:
             out["cpu_baseline"] = cpu_baseline(cfg, modes, nc, args.cpu_fraction)
         except Exception as e:  # the baseline is reported, never gating

@@ -1,4 +1,5 @@
-"""ctypes binding of libgaussnufft.so (the C ABI in include/gaussnufft.h).
+"""ctypes binding of libgaussnufft.so (the C ABI in include/gaussnufft.h and
+include/gaussnufft_solver.h).
 
 torch is imported before the library is opened so that the HIP runtime torch
 bundles is the one the library binds to.  There is no fallback: if the library
@@ -18,8 +19,10 @@ GN_OK, GN_EINVAL, GN_ENOMEM, GN_EHIP, GN_EROCFFT, GN_ESTATE = 0, 1, 2, 3, 4, 5
 GN_COIL_LAST, GN_COIL_MAJOR = 0, 1
 GN_TYPE1, GN_TYPE2, GN_ADJOINT, GN_TYPE21 = 1, 2, 3, 21
 STAGES = ("pad", "fft2", "gather", "spread", "fft1", "crop")
+GN_F32, GN_C64, GN_C128 = 1, 2, 3
+GN_TV_AMP_SQUARE, GN_TV_AMP_MODULUS = 0, 1
 
-# every symbol declared in include/gaussnufft.h, with its ctypes signature
+# every symbol declared in include/*.h, with its ctypes signature
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
@@ -40,6 +43,13 @@ SIGNATURES = {
     "gn_plan_sample_order": ([_vp, _ip], ctypes.c_int),
     "gn_traj_grad": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "gn_plan_destroy": ([_vp], ctypes.c_int),
+    # include/gaussnufft_solver.h
+    "gn_reduce_work_bytes": ([], ctypes.c_int64),
+    "gn_dot_f64": ([ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "gn_sqdist_f64": ([ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "gn_tv_work_bytes": ([ctypes.c_int, _i64p, ctypes.c_int64], ctypes.c_int64),
+    "gn_tv_prox": ([ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int64, _vp, ctypes.c_double, ctypes.c_double,
+                    ctypes.c_int, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
 }
 
 _lib = None

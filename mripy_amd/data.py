@@ -166,3 +166,15 @@ def config_3d_ute(n=128, nc=8, nspokes=4096, nread=256, seed=0):
     img = (ph[..., None] * S).astype(np.complex64)
     c = complex_normal((x.shape[0], nc), seed)
     return dict(coords=[x, y, z], image=img, samples=c, modes=(n, n, n), maps=S, phantom=ph)
+
+
+def config_2d_spiral_cs(n=256, nc=32, ninter=48, npts=2048, sigma=1e-3, seed=5):
+    """Multi-coil 2-D CS reconstruction workload: n^2 Shepp-Logan phantom, nc
+    Gaussian-blob maps, variable-density Archimedean spiral ninter x npts, complex
+    noise of std ``sigma`` from ``seed`` (added to the simulated k-space by the
+    caller, which owns the forward model)."""
+    x, y = archimedean_spiral_2d(ninter, npts, n)
+    S = coil_maps((n, n), nc, seed=seed)
+    ph = shepp_logan_2d(n)
+    noise = complex_normal((x.shape[0], nc), seed=seed, sigma=sigma)
+    return dict(coords=[x, y], maps=S, phantom=ph, noise=noise, modes=(n, n))

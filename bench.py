@@ -113,6 +113,78 @@ def cpu_baseline(cfg, modes, nc, fraction, reps=3):
     }
 
 
+def batched_uneven(dev, steps, warmup):
+    """64 uneven 2-D instances (N in {128,192,256,320}, U[32,400] spokes x 2N, U[1,32]
+    coils, default_rng(4)): fwd+adj of every instance as 64 plain plans, as one
+    batched execution, and as a batched graph replay; per-instance agreement."""
+    import numpy as np
+    import torch
+
+    from mripy_amd import data
+    from mripy_amd._lib import GN_TYPE1, GN_TYPE2
+    from mripy_amd.batch import BatchPlan
+    from mripy_amd.plan import Plan
+
+    inst = data.config_batched_uneven()
+    n = len(inst)
+    plans = [Plan(i["modes"], ncoils=i["ncoils"], rat=2, nspread=3, device=dev).set_points(i["coords"]) for i in inst]
+    imgs = [torch.as_tensor(i["image"], device=dev) for i in inst]
+
+    def bufs():
+        return ([torch.empty(p.sample_shape(), dtype=torch.complex64, device=dev) for p in plans],
+                [torch.empty(p.image_shape(), dtype=torch.complex64, device=dev) for p in plans])
+
+    ys_p, ims_p = bufs()
+
+    def plain():
+        for i in range(n):
+            plans[i].execute_into(GN_TYPE2, imgs[i], ys_p[i])
+            plans[i].execute_into(GN_TYPE1, ys_p[i], ims_p[i])
+
+    res = {}
+    out = {}
+    for name, graph in (("batched", False), ("graph", True)):
+        B = BatchPlan(inst, rat=2, nspread=3, device=dev, graph=graph)
+        ys, ims = bufs()
+
+        def run(B=B, ys=ys, ims=ims):
+            B.execute_into(GN_TYPE2, imgs, ys)
+            B.execute_into(GN_TYPE1, ys, ims)
+        out[name] = (B, ys, ims, run)
+    for name, fn in (("plain", plain), ("batched", out["batched"][3]), ("graph", out["graph"][3])):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / steps
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    worst = 0.0
+    for name in ("batched", "graph"):
+        _, ys, ims, _ = out[name]
+        for i in range(n):
+            worst = max(worst, rel(ys[i], ys_p[i]), rel(ims[i], ims_p[i]))
+    info = out["batched"][0].info()
+    coil_samples = sum(p.M * p.ncoils for p in plans)
+    return {
+        "workload": "64 uneven 2-D instances: N in {128,192,256,320}, U[32,400] golden-angle spokes x 2N, "
+                    "U[1,32] coils (default_rng(4)), rat 2, nspread 3, fwd+adj",
+        "coil_samples": coil_samples,
+        "plain_ms": res["plain"], "batched_ms": res["batched"], "graph_ms": res["graph"],
+        "speedup_batched": res["plain"] / res["batched"], "speedup_graph": res["plain"] / res["graph"],
+        "max_rel_vs_plain": worst,
+        "launches_per_fwd_adj": {"plain": n * 2 * 3, "batched": info["launches_type2"] + info["launches_type1"]},
+        "tiles": [list(info["tiles"]), list(info["gather_tiles"])],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +194,7 @@ def main():
     ap.add_argument("--cpu-fraction", type=int, default=1)
     ap.add_argument("--nspread", type=int, default=3)
     ap.add_argument("--tiles", type=str, default=None, help="spread tile shape, e.g. 4,4,8")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-instance leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 1 GiB read + 1 GiB write streaming kernel first (HBM counter calibration)")
     args = ap.parse_args()
@@ -229,6 +302,14 @@ def main():
         "stages": stages,
         "wall_s_timed": wall,
     }
+    if not args.no_batched:
+        log("batched uneven instances")
+        del ybuf, obuf, img, P
+        torch.cuda.empty_cache()
+        out["batched_uneven"] = batched_uneven(dev, steps=max(3, args.steps // 4), warmup=2)
+        log("batched: plain %.2f ms, batched %.2f ms, graph %.2f ms"
+            % (out["batched_uneven"]["plain_ms"], out["batched_uneven"]["batched_ms"],
+               out["batched_uneven"]["graph_ms"]))
     if not args.no_cpu:
         log("CPU baseline")
         try:

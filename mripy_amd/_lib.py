@@ -52,6 +52,31 @@ SIGNATURES = {
                     ctypes.c_int, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
 }
 
+
+
+class GNInstance(ctypes.Structure):
+    """gn_instance of include/gaussnufft_batch.h"""
+    _fields_ = [("modes", ctypes.c_int64 * 3), ("ncoils", ctypes.c_int64), ("M", ctypes.c_int64),
+                ("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("z", ctypes.c_void_p), ("dcf", ctypes.c_void_p)]
+
+
+_vpp = ctypes.POINTER(ctypes.c_void_p)
+SIGNATURES.update({
+    # include/gaussnufft_batch.h
+    "gn_batch_create": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(GNInstance), ctypes.c_int, ctypes.c_int,
+                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, ctypes.POINTER(_vp)],
+                        ctypes.c_int),
+    "gn_batch_execute": ([_vp, ctypes.c_int, _vpp, _vpp], ctypes.c_int),
+    "gn_batch_set_graph": ([_vp, ctypes.c_int], ctypes.c_int),
+    "gn_batch_set_tiles": ([_vp, _ip], ctypes.c_int),
+    "gn_batch_autotune": ([_vp, ctypes.c_int, _vpp, _vpp, ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_double)],
+                          ctypes.c_int),
+    "gn_batch_info": ([_vp, _ip, _ip, _ip], ctypes.c_int),
+    "gn_batch_destroy": ([_vp], ctypes.c_int),
+    "gn_plan_autotune": ([_vp, ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp,
+                          _vp, ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+})
+
 _lib = None
 
 

@@ -221,10 +221,15 @@ int gn_grid_params(int dim, const int64_t *modes, double eps, int max_nspread, i
     return GN_OK;
 }
 
-int gn_plan_create(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,
-                   void *stream, gn_plan **plan)
+} // extern "C"
+
+namespace gn {
+
+// Plan set-up shared by gn_plan_create and batch members.  with_grid: allocate the
+// oversampled grid and create the rocFFT plans (batch members use the batch's).
+int plan_create_impl(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,
+                     void *stream, bool with_grid, gn_plan_s **plan)
 {
-    g_err.clear();
     if (!plan) return fail(GN_EINVAL, "plan pointer is null");
     *plan = nullptr;
     if (dim != 2 && dim != 3) return fail(GN_EINVAL, "dim must be 2 or 3");
@@ -275,7 +280,7 @@ int gn_plan_create(int dim, const int64_t *modes, int rat, int nspread, int ifla
         }
     if (hipMalloc((void **)&p->deap, deap.size() * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&p->err_flag, sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&p->grid, (size_t)g.G * ncoils * sizeof(float2)) != hipSuccess) {
+        (with_grid && hipMalloc((void **)&p->grid, (size_t)g.G * ncoils * sizeof(float2)) != hipSuccess)) {
         gn_plan_destroy(p);
         return fail(GN_ENOMEM, "hipMalloc failed for the oversampled grid");
     }
@@ -284,15 +289,28 @@ int gn_plan_create(int dim, const int64_t *modes, int rat, int nspread, int ifla
         gn_plan_destroy(p);
         return hip_fail(e, "deapodization upload");
     }
-    rc = create_fft(p);
-    if (rc) {
-        std::string keep = g_err;
-        gn_plan_destroy(p);
-        g_err = keep;
-        return rc;
+    if (with_grid) {
+        rc = create_fft(p);
+        if (rc) {
+            std::string keep = g_err;
+            gn_plan_destroy(p);
+            g_err = keep;
+            return rc;
+        }
     }
     *plan = p;
     return GN_OK;
+}
+
+} // namespace gn
+
+extern "C" {
+
+int gn_plan_create(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,
+                   void *stream, gn_plan **plan)
+{
+    g_err.clear();
+    return plan_create_impl(dim, modes, rat, nspread, iflag, ncoils, layout, stream, true, plan);
 }
 
 int gn_plan_set_stream(gn_plan *p, void *stream)

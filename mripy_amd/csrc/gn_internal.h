@@ -62,6 +62,63 @@ struct Geometry {
     float coef[3]; // log2(e) * h_a^2/(4 tau): weight = exp2(-coef * (t + ns - j)^2)
 };
 
+// ---- kernel arguments (shared by the plan and batch launchers)
+
+struct SpreadArgs {
+    int64_t nf[3];
+    int tile[3];
+    int ntiles[3];
+    int blk[3];  // block (cells per wave) per axis
+    int nblk[3]; // blocks per tile per axis
+    int64_t G;
+    int64_t M;
+    int64_t nc;
+    int TC;
+    int TT;      // tile[0] + tile[1] + tile[2]: weight-table entries per pair
+    int ns;
+    int layout;
+    int use_dcf;
+    float coef[3];
+    const int4 *items;
+    const int4 *reduce_items;
+    const int4 *pair_a;
+    const int4 *pair_b;
+    const float2 *in;
+    float2 *grid;
+    float2 *partial;
+};
+
+struct GatherArgs {
+    int64_t nf[3];
+    int tile[3];
+    int ntiles[3];
+    int region[3]; // staged cells per axis
+    int64_t G;
+    int64_t M;
+    int64_t nc;
+    int layout;
+    float coef[3];
+    const int4 *items;
+    const int4 *rec_cell;
+    const float4 *rec_frac;
+    const float2 *grid;
+    float2 *out;
+};
+
+struct PrepostArgs {
+    int dim;
+    int64_t modes[3];
+    int64_t nf[3];
+    int64_t G;
+    int64_t N;
+    int64_t nc;
+    int layout;
+    float scale;
+    const float *deap0, *deap1, *deap2;
+    const float2 *src; // batched launches: this instance's input
+    float2 *dst;       // and output
+};
+
 } // namespace gn
 
 struct gn_plan_s {
@@ -120,5 +177,23 @@ int launch_pad(gn_plan_s *p, const float2 *image, float scale);
 int launch_crop(gn_plan_s *p, float2 *image, float scale);
 int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out);
 void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile);
+
+// api.hip
+int plan_create_impl(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,
+                     void *stream, bool with_grid, gn_plan_s **plan);
+
+// argument builders and batched launchers (kernels.hip)
+SpreadArgs make_spread_args(const gn_plan_s *p, const float2 *samples, int use_dcf);
+GatherArgs make_gather_args(const gn_plan_s *p, float2 *samples);
+PrepostArgs make_prepost(const gn_plan_s *p, float scale);
+void spread_shape(const SpreadArgs &a, int dim, int *waves, int *nbg);
+size_t spread_lds(const SpreadArgs &a, int ncg);
+hipError_t launch_spread_batch(int dim, int ncg, const SpreadArgs *d_args, const int4 *d_items, int n, int waves,
+                               int nbg, size_t lds, hipStream_t st);
+hipError_t launch_spread_reduce_batch(int dim, const SpreadArgs *d_args, const int4 *d_items, int n, hipStream_t st);
+hipError_t launch_gather_batch(int dim, int ns, bool wrap, const GatherArgs *d_args, const int4 *d_items, int n,
+                               size_t lds, hipStream_t st);
+hipError_t launch_pad_batch(int dim, const PrepostArgs *d_args, int ninst, int64_t maxG, hipStream_t st);
+hipError_t launch_crop_batch(int dim, const PrepostArgs *d_args, int ninst, int64_t maxN, hipStream_t st);
 
 } // namespace gn

@@ -391,34 +391,13 @@ __global__ __launch_bounds__(256) void k_permute_pairs(int64_t n, const int *__r
 // split tile go to partial buffers combined by k_spread_reduce.
 // ---------------------------------------------------------------------------
 
-struct SpreadArgs {
-    int64_t nf[3];
-    int tile[3];
-    int ntiles[3];
-    int blk[3];  // block (cells per wave) per axis
-    int nblk[3]; // blocks per tile per axis
-    int64_t G;
-    int64_t M;
-    int64_t nc;
-    int TC;
-    int TT;      // tile[0] + tile[1] + tile[2]: weight-table entries per pair
-    int ns;
-    int layout;
-    int use_dcf;
-    float coef[3];
-    const int4 *items;
-    const int4 *pair_a;
-    const int4 *pair_b;
-    const float2 *in;
-    float2 *grid;
-    float2 *partial;
-};
+
 
 constexpr int kSpreadBatch = 64;
 constexpr int kSpreadWavesMax = 8;
 
 template <int D, int NCG>
-__global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
+__device__ inline void spread_body(const SpreadArgs &a, const int4 item, const int cgi, const int bz)
 {
     extern __shared__ float lds_sp[];
     // LDS: weight tables [batch][TT] | coil values [batch][NCG] float2 | masks [batch][3]
@@ -432,14 +411,13 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
     const int wave = tid >> 6;
     const int lane = tid & 63;
     const int nwaves = blockDim.x >> 6;
-    const int4 item = a.items[blockIdx.x];
-    const int cg0 = blockIdx.y * NCG;
+    const int cg0 = cgi * NCG;
     const int ncg = (int)min((int64_t)NCG, a.nc - cg0);
     const int T0 = a.tile[0], T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
 
     // this wave's block (block index = blockIdx.z * waves + wave) and this lane's cell
     const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
-    const int bidx = blockIdx.z * nwaves + wave;
+    const int bidx = bz * nwaves + wave;
     const bool has_block = bidx < nb;
     int bq0, bq1, bq2 = 0;
     if (D == 3) {
@@ -623,11 +601,27 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
         if (c < ncg) a.grid[(int64_t)(cg0 + c) * a.G + gi] = make_float2(sr[c] - kr[c], si[c] - ki[c]);
 }
 
+template <int D, int NCG>
+__global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
+{
+    spread_body<D, NCG>(a, a.items[blockIdx.x], blockIdx.y, blockIdx.z);
+}
+
+// Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0}.
+template <int D, int NCG>
+__global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const SpreadArgs *__restrict__ args,
+                                                                        const int4 *__restrict__ bitems)
+{
+    const int4 bi = bitems[blockIdx.x];
+    const SpreadArgs a = args[bi.x];
+    spread_body<D, NCG>(a, a.items[bi.y], bi.z, blockIdx.y);
+}
+
 // Kahan-compensated sum of the partial tiles of each split tile, in slice order.
 template <int D>
-__global__ __launch_bounds__(256) void k_spread_reduce(SpreadArgs a, const int4 *__restrict__ ritems)
+__device__ inline void spread_reduce_body(const SpreadArgs &a, const int4 it)
 {
-    const int4 it = ritems[blockIdx.x]; // {tile, slot0, nslots}
+    // it = {tile, slot0, nslots}
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     {
         int64_t r = it.x;
@@ -669,6 +663,22 @@ __global__ __launch_bounds__(256) void k_spread_reduce(SpreadArgs a, const int4 
     }
 }
 
+template <int D>
+__global__ __launch_bounds__(256) void k_spread_reduce(SpreadArgs a, const int4 *__restrict__ ritems)
+{
+    spread_reduce_body<D>(a, ritems[blockIdx.x]);
+}
+
+// bitems[b] = {instance, reduce item}
+template <int D>
+__global__ __launch_bounds__(256) void k_spread_reduce_batch(const SpreadArgs *__restrict__ args,
+                                                             const int4 *__restrict__ bitems)
+{
+    const int4 bi = bitems[blockIdx.x];
+    const SpreadArgs a = args[bi.x];
+    spread_reduce_body<D>(a, a.reduce_items[bi.y]);
+}
+
 // ---------------------------------------------------------------------------
 // Forward gather: one workgroup per work item (a gather tile and up to
 // kGatherChunk of its samples).  The tile plus its window halo is staged in LDS
@@ -676,30 +686,14 @@ __global__ __launch_bounds__(256) void k_spread_reduce(SpreadArgs a, const int4 
 // samples from LDS, forming the separable 1-D weights in registers.
 // ---------------------------------------------------------------------------
 
-struct GatherArgs {
-    int64_t nf[3];
-    int tile[3];
-    int ntiles[3];
-    int region[3]; // staged cells per axis
-    int64_t G;
-    int64_t M;
-    int64_t nc;
-    int layout;
-    float coef[3];
-    const int4 *items;
-    const int4 *rec_cell;
-    const float4 *rec_frac;
-    const float2 *grid;
-    float2 *out;
-};
+
 
 template <int D, int NS, bool WRAP>
-__global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
+__device__ inline void gather_body(const GatherArgs &a, const int4 item, const int cgi)
 {
     constexpr int W = 2 * NS;
     extern __shared__ float2 lds_gather[];
-    const int4 item = a.items[blockIdx.x];
-    const int cg0 = blockIdx.y * kCoilGroup;
+    const int cg0 = cgi * kCoilGroup;
     const int ncg = (int)min((int64_t)kCoilGroup, a.nc - cg0);
 
     int org[3] = {0, 0, 0};
@@ -798,6 +792,22 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
                 a.out[coil * a.M + rc.w] = make_float2(ar, ai);
         }
     }
+}
+
+template <int D, int NS, bool WRAP>
+__global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
+{
+    gather_body<D, NS, WRAP>(a, a.items[blockIdx.x], blockIdx.y);
+}
+
+// Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0}.
+template <int D, int NS, bool WRAP>
+__global__ __launch_bounds__(kGatherThreads) void k_gather_batch(const GatherArgs *__restrict__ args,
+                                                                 const int4 *__restrict__ bitems)
+{
+    const int4 bi = bitems[blockIdx.x];
+    const GatherArgs a = args[bi.x];
+    gather_body<D, NS, WRAP>(a, a.items[bi.y], bi.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -936,20 +946,10 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
 // grid index k mod nf (fft/nufft_func.py:761-764, 728-729).
 // ---------------------------------------------------------------------------
 
-struct PrepostArgs {
-    int dim;
-    int64_t modes[3];
-    int64_t nf[3];
-    int64_t G;
-    int64_t N;
-    int64_t nc;
-    int layout;
-    float scale;
-    const float *deap0, *deap1, *deap2;
-};
+
 
 template <int D>
-__global__ __launch_bounds__(256) void k_pad(PrepostArgs a, const float2 *__restrict__ img, float2 *__restrict__ grid)
+__device__ inline void pad_body(const PrepostArgs &a, const float2 *__restrict__ img, float2 *__restrict__ grid)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t gidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < a.G; gidx += stride) {
@@ -992,7 +992,21 @@ __global__ __launch_bounds__(256) void k_pad(PrepostArgs a, const float2 *__rest
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_crop(PrepostArgs a, const float2 *__restrict__ grid, float2 *__restrict__ img)
+__global__ __launch_bounds__(256) void k_pad(PrepostArgs a, const float2 *__restrict__ img, float2 *__restrict__ grid)
+{
+    pad_body<D>(a, img, grid);
+}
+
+// batched: blockIdx.y = instance; args[i].src = image, args[i].dst = grid
+template <int D>
+__global__ __launch_bounds__(256) void k_pad_batch(const PrepostArgs *__restrict__ args)
+{
+    const PrepostArgs a = args[blockIdx.y];
+    pad_body<D>(a, a.src, a.dst);
+}
+
+template <int D>
+__device__ inline void crop_body(const PrepostArgs &a, const float2 *__restrict__ grid, float2 *__restrict__ img)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += stride) {
@@ -1027,6 +1041,20 @@ __global__ __launch_bounds__(256) void k_crop(PrepostArgs a, const float2 *__res
                 img[c * a.N + n] = o;
         }
     }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_crop(PrepostArgs a, const float2 *__restrict__ grid, float2 *__restrict__ img)
+{
+    crop_body<D>(a, grid, img);
+}
+
+// batched: blockIdx.y = instance; args[i].src = grid, args[i].dst = image
+template <int D>
+__global__ __launch_bounds__(256) void k_crop_batch(const PrepostArgs *__restrict__ args)
+{
+    const PrepostArgs a = args[blockIdx.y];
+    crop_body<D>(a, a.src, a.dst);
 }
 
 inline int grid_blocks(int64_t n, int threads, int64_t cap = 1 << 20)
@@ -1408,7 +1436,7 @@ hipError_t gather_any(int dim, int ns, const GatherArgs &a, int nitems, int64_t 
     return hipErrorInvalidValue;
 }
 
-PrepostArgs make_prepost(const gn_plan_s *p, float scale)
+PrepostArgs make_prepost_impl(const gn_plan_s *p, float scale)
 {
     const Geometry &g = p->geo;
     PrepostArgs a;
@@ -1425,12 +1453,14 @@ PrepostArgs make_prepost(const gn_plan_s *p, float scale)
     a.deap0 = p->deap;
     a.deap1 = p->deap + g.modes[0];
     a.deap2 = p->deap + g.modes[0] + g.modes[1];
+    a.src = nullptr;
+    a.dst = nullptr;
     return a;
 }
 
 } // namespace
 
-int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
+SpreadArgs make_spread_args(const gn_plan_s *p, const float2 *samples, int use_dcf)
 {
     const Geometry &g = p->geo;
     SpreadArgs a;
@@ -1453,21 +1483,28 @@ int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
     a.layout = p->layout;
     a.use_dcf = use_dcf && p->has_dcf;
     a.items = p->rec.spread_items;
+    a.reduce_items = p->rec.reduce_items;
     a.pair_a = p->rec.pair_a;
     a.pair_b = p->rec.pair_b;
     a.in = samples;
     a.grid = p->grid;
     a.partial = p->rec.partial;
+    return a;
+}
+
+int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
+{
+    const Geometry &g = p->geo;
+    const SpreadArgs a = make_spread_args(p, samples, use_dcf);
     hipError_t e = spread_any(g.dim, g.ns, a, p->rec.n_spread_items, p->rec.n_reduce_items, p->rec.reduce_items, p->nc,
                               p->stream);
     if (e != hipSuccess) return hip_fail(e, "spread launch");
     return GN_OK;
 }
 
-int launch_gather(gn_plan_s *p, float2 *samples)
+GatherArgs make_gather_args(const gn_plan_s *p, float2 *samples)
 {
     const Geometry &g = p->geo;
-    if (p->M == 0 || p->rec.n_gather_items == 0) return GN_OK;
     GatherArgs a;
     for (int ax = 0; ax < 3; ++ax) {
         a.nf[ax] = g.nf[ax];
@@ -1485,6 +1522,14 @@ int launch_gather(gn_plan_s *p, float2 *samples)
     a.rec_frac = p->rec.frac;
     a.grid = p->grid;
     a.out = samples;
+    return a;
+}
+
+int launch_gather(gn_plan_s *p, float2 *samples)
+{
+    const Geometry &g = p->geo;
+    if (p->M == 0 || p->rec.n_gather_items == 0) return GN_OK;
+    const GatherArgs a = make_gather_args(p, samples);
     hipError_t e = gather_any(g.dim, g.ns, a, p->rec.n_gather_items, p->nc, p->stream);
     if (e != hipSuccess) return hip_fail(e, "gather launch");
     return GN_OK;
@@ -1552,6 +1597,8 @@ int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out)
     return GN_OK;
 }
 
+PrepostArgs make_prepost(const gn_plan_s *p, float scale) { return make_prepost_impl(p, scale); }
+
 int launch_pad(gn_plan_s *p, const float2 *image, float scale)
 {
     PrepostArgs a = make_prepost(p, scale);
@@ -1576,6 +1623,114 @@ int launch_crop(gn_plan_s *p, float2 *image, float scale)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "crop launch");
     return GN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched launchers: one launch per stage over the work of many plans.
+// ---------------------------------------------------------------------------
+
+void spread_shape(const SpreadArgs &a, int dim, int *waves, int *nbg)
+{
+    const int nb = a.nblk[0] * a.nblk[1] * (dim == 3 ? a.nblk[2] : 1);
+    *waves = std::min(nb, kSpreadWavesMax);
+    *nbg = (nb + *waves - 1) / *waves;
+}
+
+size_t spread_lds(const SpreadArgs &a, int ncg) { return spread_lds_bytes(a, ncg); }
+
+namespace {
+template <int D, int NCG>
+hipError_t spread_batch_launch(const SpreadArgs *d_args, const int4 *d_items, int n, int waves, int nbg, size_t lds,
+                               hipStream_t st)
+{
+    lds_limit(k_spread_batch<D, NCG>, lds);
+    hipLaunchKernelGGL((k_spread_batch<D, NCG>), dim3((unsigned)n, (unsigned)nbg), dim3(64 * waves), lds, st, d_args,
+                       d_items);
+    return hipGetLastError();
+}
+
+template <int D, int NS>
+hipError_t gather_batch_dispatch(bool wrap, const GatherArgs *d_args, const int4 *d_items, int n, size_t lds,
+                                 hipStream_t st)
+{
+    if (wrap) {
+        lds_limit(k_gather_batch<D, NS, true>, lds);
+        hipLaunchKernelGGL((k_gather_batch<D, NS, true>), dim3((unsigned)n), dim3(kGatherThreads), lds, st, d_args,
+                           d_items);
+    } else {
+        lds_limit(k_gather_batch<D, NS, false>, lds);
+        hipLaunchKernelGGL((k_gather_batch<D, NS, false>), dim3((unsigned)n), dim3(kGatherThreads), lds, st, d_args,
+                           d_items);
+    }
+    return hipGetLastError();
+}
+} // namespace
+
+hipError_t launch_spread_batch(int dim, int ncg, const SpreadArgs *d_args, const int4 *d_items, int n, int waves,
+                               int nbg, size_t lds, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (dim == 3) {
+        switch (ncg) {
+        case 8: return spread_batch_launch<3, 8>(d_args, d_items, n, waves, nbg, lds, st);
+        case 4: return spread_batch_launch<3, 4>(d_args, d_items, n, waves, nbg, lds, st);
+        case 2: return spread_batch_launch<3, 2>(d_args, d_items, n, waves, nbg, lds, st);
+        case 1: return spread_batch_launch<3, 1>(d_args, d_items, n, waves, nbg, lds, st);
+        }
+    } else {
+        switch (ncg) {
+        case 8: return spread_batch_launch<2, 8>(d_args, d_items, n, waves, nbg, lds, st);
+        case 4: return spread_batch_launch<2, 4>(d_args, d_items, n, waves, nbg, lds, st);
+        case 2: return spread_batch_launch<2, 2>(d_args, d_items, n, waves, nbg, lds, st);
+        case 1: return spread_batch_launch<2, 1>(d_args, d_items, n, waves, nbg, lds, st);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_spread_reduce_batch(int dim, const SpreadArgs *d_args, const int4 *d_items, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (dim == 3)
+        hipLaunchKernelGGL(k_spread_reduce_batch<3>, dim3((unsigned)n), dim3(256), 0, st, d_args, d_items);
+    else
+        hipLaunchKernelGGL(k_spread_reduce_batch<2>, dim3((unsigned)n), dim3(256), 0, st, d_args, d_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_batch(int dim, int ns, bool wrap, const GatherArgs *d_args, const int4 *d_items, int n,
+                               size_t lds, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (dim == 3) {
+        GN_NS_CASES(3, gather_batch_dispatch, wrap, d_args, d_items, n, lds, st)
+    } else {
+        GN_NS_CASES(2, gather_batch_dispatch, wrap, d_args, d_items, n, lds, st)
+        GN_NS_CASES_2D_EXTRA(gather_batch_dispatch, wrap, d_args, d_items, n, lds, st)
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pad_batch(int dim, const PrepostArgs *d_args, int ninst, int64_t maxG, hipStream_t st)
+{
+    if (ninst <= 0) return hipSuccess;
+    dim3 grid(grid_blocks(maxG, 256, 1 << 14), (unsigned)ninst);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_pad_batch<3>, grid, dim3(256), 0, st, d_args);
+    else
+        hipLaunchKernelGGL(k_pad_batch<2>, grid, dim3(256), 0, st, d_args);
+    return hipGetLastError();
+}
+
+hipError_t launch_crop_batch(int dim, const PrepostArgs *d_args, int ninst, int64_t maxN, hipStream_t st)
+{
+    if (ninst <= 0) return hipSuccess;
+    dim3 grid(grid_blocks(maxN, 256, 1 << 14), (unsigned)ninst);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_crop_batch<3>, grid, dim3(256), 0, st, d_args);
+    else
+        hipLaunchKernelGGL(k_crop_batch<2>, grid, dim3(256), 0, st, d_args);
+    return hipGetLastError();
 }
 
 } // namespace gn

@@ -178,3 +178,23 @@ def config_2d_spiral_cs(n=256, nc=32, ninter=48, npts=2048, sigma=1e-3, seed=5):
     ph = shepp_logan_2d(n)
     noise = complex_normal((x.shape[0], nc), seed=seed, sigma=sigma)
     return dict(coords=[x, y], maps=S, phantom=ph, noise=noise, modes=(n, n))
+
+
+def config_batched_uneven(ninst=64, seed=4, sizes=(128, 192, 256, 320), spokes=(32, 400), coils=(1, 32),
+                          with_images=True):
+    """Uneven 2-D multi-instance workload: per instance N drawn from ``sizes``, a
+    golden-angle radial trajectory of U[spokes] spokes x 2N readout points, U[coils]
+    coils with smooth Gaussian-blob maps, image = Shepp-Logan x maps (default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(ninst):
+        N = int(rng.choice(sizes))
+        nsp = int(rng.integers(spokes[0], spokes[1] + 1))
+        nc = int(rng.integers(coils[0], coils[1] + 1))
+        x, y = golden_angle_radial_2d(nsp, 2 * N)
+        inst = dict(modes=(N, N), coords=[x, y], ncoils=nc)
+        if with_images:
+            S = coil_maps((N, N), nc, seed=seed * 1000 + i)
+            inst["image"] = (shepp_logan_2d(N)[..., None] * S).astype(np.complex64)
+        out.append(inst)
+    return out

@@ -132,6 +132,7 @@ class Plan:
         self.M = M
         self._dcf = d
         self._points = [c for c in coords]
+        self._bound = (ts, f64, float(df), d)   # device copies, for re-binning by autotune
         return self
 
     # ------------------------------------------------------------------ shapes
@@ -184,6 +185,22 @@ class Plan:
         """Run on preallocated contiguous complex64 device tensors (no allocation, no checks of shape)."""
         check(_lib.load().gn_execute(self._h, kind, src.data_ptr(), dst.data_ptr()))
         return dst
+
+    def autotune(self, kind, src, dst, reps=3):
+        """Time every candidate tile shape on ``kind`` (src/dst: contiguous complex64 device
+        tensors) and keep the fastest; returns ((spread tiles, gather tiles), ms per run)."""
+        if self.M is None:
+            raise RuntimeError("set_points must be called first")
+        ts, f64, df, d = self._bound
+        t = (ctypes.c_int * 6)()
+        ms = ctypes.c_double()
+        with torch.cuda.device(self.device):
+            self._bind_stream()
+            z = ts[2].data_ptr() if self.dim == 3 else None
+            check(_lib.load().gn_plan_autotune(self._h, kind, self.M, ts[0].data_ptr(), ts[1].data_ptr(), z,
+                                               1 if f64 else 0, df, d.data_ptr() if d is not None else None,
+                                               src.data_ptr(), dst.data_ptr(), int(reps), t, ctypes.byref(ms)))
+        return (tuple(t[:self.dim]), tuple(t[3:3 + self.dim])), ms.value
 
     def traj_grad(self, image, weights):
         """(dim, M) float32: sum_c Re(conj(w) d(type2 image)/dx) per sample and axis (analytic)."""

@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--nspread", type=int, default=3)
     ap.add_argument("--tiles", type=str, default=None, help="spread tile shape, e.g. 4,4,8")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-instance leg")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched uneven-instance leg (prints its JSON object)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 1 GiB read + 1 GiB write streaming kernel first (HBM counter calibration)")
     args = ap.parse_args()
@@ -211,6 +213,13 @@ def main():
 
     from mripy_amd import data
     from mripy_amd.plan import Plan
+
+    if args.only_batched:
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        log("batched uneven instances only")
+        print(json.dumps(batched_uneven(dev, steps=args.steps, warmup=args.warmup)))
+        return
 
     modes, nc, nspokes, nread = (128, 128, 128), 8, 4096, 256
     log("generating the 3-D UTE workload")

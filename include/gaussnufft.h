@@ -57,6 +57,11 @@ extern "C" {
 #define GN_ADJOINT 3 /* exact adjoint of GN_TYPE2: M * GN_TYPE1 without dcf                  */
 #define GN_TYPE21 21 /* GN_TYPE1(GN_TYPE2(image)) with dcf                                   */
 
+/* FFT engines for gn_plan_set_fft_engine */
+#define GN_FFT_AUTO 0   /* pruned in-place passes fused with pad/crop when every nf is a product of
+                           2, 3 and 5 (<= 2048); rocFFT otherwise */
+#define GN_FFT_ROCFFT 1 /* separate pad, rocFFT and crop */
+
 typedef struct gn_plan_s gn_plan;
 
 /* Library version (GN_VERSION). */
@@ -132,6 +137,9 @@ int gn_plan_stage_times(gn_plan *plan, double *ms, int64_t *counts, int reset);
  * original sample index; M ints) to host memory.  No counterpart in the
  * reference: it exposes the stable counting-sort result to tests and tools. */
 int gn_plan_sample_order(const gn_plan *plan, int *order_host);
+
+/* select the FFT engine (GN_FFT_AUTO or GN_FFT_ROCFFT); results agree to FP32 roundoff */
+int gn_plan_set_fft_engine(gn_plan *plan, int engine);
 
 int gn_plan_destroy(gn_plan *plan);
 

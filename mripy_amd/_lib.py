@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libgaussnufft.so")
 GN_OK, GN_EINVAL, GN_ENOMEM, GN_EHIP, GN_EROCFFT, GN_ESTATE = 0, 1, 2, 3, 4, 5
 GN_COIL_LAST, GN_COIL_MAJOR = 0, 1
 GN_TYPE1, GN_TYPE2, GN_ADJOINT, GN_TYPE21 = 1, 2, 3, 21
+GN_FFT_AUTO, GN_FFT_ROCFFT = 0, 1
 STAGES = ("pad", "fft2", "gather", "spread", "fft1", "crop")
 GN_F32, GN_C64, GN_C128 = 1, 2, 3
 GN_TV_AMP_SQUARE, GN_TV_AMP_MODULUS = 0, 1
@@ -42,6 +43,7 @@ SIGNATURES = {
     "gn_plan_stage_times": ([_vp, ctypes.POINTER(ctypes.c_double), _i64p, ctypes.c_int], ctypes.c_int),
     "gn_plan_sample_order": ([_vp, _ip], ctypes.c_int),
     "gn_traj_grad": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "gn_plan_set_fft_engine": ([_vp, ctypes.c_int], ctypes.c_int),
     "gn_plan_destroy": ([_vp], ctypes.c_int),
     # include/gaussnufft_solver.h
     "gn_reduce_work_bytes": ([], ctypes.c_int64),

@@ -290,7 +290,10 @@ int plan_create_impl(int dim, const int64_t *modes, int rat, int nspread, int if
         return hip_fail(e, "deapodization upload");
     }
     if (with_grid) {
-        rc = create_fft(p);
+        p->fft_custom = true;
+        for (int a = 0; a < dim; ++a) p->fft_custom &= fft_length_supported(g.nf[a]);
+        for (int a = 0; a < dim && p->fft_custom && !rc; ++a) rc = fft_axis_init(&p->fft_ax[a], (int)g.nf[a]);
+        if (!rc && !p->fft_custom) rc = create_fft(p);
         if (rc) {
             std::string keep = g_err;
             gn_plan_destroy(p);
@@ -388,18 +391,36 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
     // type 2 uses e^{-ik.x} for iflag >= 0; type 1 the opposite sign
     const bool fwd_dir_type2 = p->iflag >= 0;
     int rc = GN_OK;
+    // pruned passes (sign of the type-2 FFT: -1 for iflag >= 0) or pad / rocFFT / crop
+    const bool custom = p->fft_custom && p->fft_engine == GN_FFT_AUTO;
+    if (!custom && !p->fft_fwd) {
+        int r = create_fft(p);
+        if (r) return r;
+    }
+    const float sign2 = fwd_dir_type2 ? -1.0f : 1.0f;
     auto type2 = [&](const void *img, void *samples) -> int {
-        int r = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)img, (float)(c / (double)g.G)); });
-        if (!r) r = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, fwd_dir_type2); });
+        const float scale = (float)(c / (double)g.G);
+        int r;
+        if (custom) {
+            r = timed(p, GN_STAGE_PAD, [&] { return pruned_type2_rows(p, (const float2 *)img, scale, sign2); });
+            if (!r) r = timed(p, GN_STAGE_FFT2, [&] { return pruned_type2_cols(p, sign2); });
+        } else {
+            r = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)img, scale); });
+            if (!r) r = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, fwd_dir_type2); });
+        }
         if (!r) r = timed(p, GN_STAGE_GATHER, [&] { return launch_gather(p, (float2 *)samples); });
         return r;
     };
     auto type1 = [&](const void *samples, void *img, bool dcf, double extra) -> int {
+        const float scale = (float)(c * extra / (double)g.G);
         int r = timed(p, GN_STAGE_SPREAD, [&] { return launch_spread(p, (const float2 *)samples, dcf ? 1 : 0); });
-        if (!r) r = timed(p, GN_STAGE_FFT1, [&] { return run_fft(p, !fwd_dir_type2); });
-        if (!r)
-            r = timed(p, GN_STAGE_CROP,
-                      [&] { return launch_crop(p, (float2 *)img, (float)(c * extra / (double)g.G)); });
+        if (custom) {
+            if (!r) r = timed(p, GN_STAGE_FFT1, [&] { return pruned_type1_cols(p, -sign2); });
+            if (!r) r = timed(p, GN_STAGE_CROP, [&] { return pruned_type1_rows(p, (float2 *)img, scale, -sign2); });
+        } else {
+            if (!r) r = timed(p, GN_STAGE_FFT1, [&] { return run_fft(p, !fwd_dir_type2); });
+            if (!r) r = timed(p, GN_STAGE_CROP, [&] { return launch_crop(p, (float2 *)img, scale); });
+        }
         return r;
     };
     switch (kind) {
@@ -439,8 +460,17 @@ int gn_traj_grad(gn_plan *p, const void *image, const void *weights, float *grad
     const Geometry &g = p->geo;
     const double pi = 3.141592653589793;
     const double c = std::pow(pi / p->tau, g.dim / 2.0);
-    int rc = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)image, (float)(c / (double)g.G)); });
-    if (!rc) rc = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, p->iflag >= 0); });
+    const float scale = (float)(c / (double)g.G);
+    int rc;
+    if (p->fft_custom && p->fft_engine == GN_FFT_AUTO) {
+        const float sign2 = p->iflag >= 0 ? -1.0f : 1.0f;
+        rc = timed(p, GN_STAGE_PAD, [&] { return pruned_type2_rows(p, (const float2 *)image, scale, sign2); });
+        if (!rc) rc = timed(p, GN_STAGE_FFT2, [&] { return pruned_type2_cols(p, sign2); });
+    } else {
+        rc = p->fft_fwd ? GN_OK : create_fft(p);
+        if (!rc) rc = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)image, scale); });
+        if (!rc) rc = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, p->iflag >= 0); });
+    }
     if (!rc) rc = launch_traj_grad(p, (const float2 *)weights, grad);
     return rc;
 }
@@ -512,6 +542,15 @@ int gn_plan_sample_order(const gn_plan *p, int *order)
     return GN_OK;
 }
 
+int gn_plan_set_fft_engine(gn_plan *p, int engine)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    if (engine != GN_FFT_AUTO && engine != GN_FFT_ROCFFT) return fail(GN_EINVAL, "unknown FFT engine");
+    p->fft_engine = engine;
+    return GN_OK;
+}
+
 int gn_plan_destroy(gn_plan *p)
 {
     if (!p) return GN_OK;
@@ -522,6 +561,7 @@ int gn_plan_destroy(gn_plan *p)
     }
     for (auto e : p->event_pool) hipEventDestroy(e);
     destroy_fft(p);
+    for (auto &ax : p->fft_ax) fft_axis_free(&ax);
     hipFree(p->rec.cell);
     hipFree(p->rec.frac);
     hipFree(p->rec.bin_start);

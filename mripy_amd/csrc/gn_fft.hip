@@ -1,0 +1,544 @@
+// Pruned oversampled-grid FFTs fused with the pre/post passes.
+//
+// Type 2 pads an m-point image axis into nf cells (frequency k = n - m/2 at cell
+// (k mod nf)) and transforms every axis; type 1 transforms and keeps the same m
+// cells.  Done axis by axis, every pass touches only the cells that are non-zero
+// (type 2) or needed (type 1) on the axes not yet transformed:
+//
+//   type 2, 3-D: A  z-rows of the m0 x m1 image lines      image -> cells (n0 < m0, n1 < m1, all k2)
+//                B  y-columns, m1 rows in -> nf1 out       -> cells (n0 < m0, all k1, all k2)
+//                C  x-columns, m0 rows in -> nf0 out       -> every cell
+//   type 1, 3-D: C' x-columns, nf0 rows in -> m0 kept       -> cells (n0 < m0, ...)
+//                B' y-columns, nf1 rows in -> m1 kept       -> cells (n0 < m0, n1 < m1, ...)
+//                A' z-rows, nf2 in -> m2 kept               -> image
+//
+// (2-D: passes A and C only.)  Every pass works in place in the plan's grid buffer:
+// a block reads all rows of its (slab, column block) into LDS before it writes
+// any, and no two blocks share an address, so the partial transforms live in the
+// leading rows of the grid itself and no scratch buffer is needed.  Deapodization and scale are applied in A (type 2)
+// and A' (type 1), which also read / write the caller's coil-last or coil-major
+// image.  Each 1-D FFT runs in LDS as an in-place mixed-radix (2, 3, 4, 5)
+// decimation-in-time transform: loads scatter the input to digit-reversed slots,
+// so the stages need no reordering and the output comes out in natural order.
+// Twiddles come from an FP64-computed table of e^{-2 pi i k / L}.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gn_internal.h"
+
+namespace gn {
+namespace {
+
+constexpr int kFFTThreads = 256;
+
+struct Stages {
+    int n;
+    int radix[kFFTMaxStages];
+};
+
+__device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by (sign * i)
+__device__ inline float2 mul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// One radix-r butterfly on LDS elements at idx0 + k*step_q (k < r) of a line, after
+// twiddling element k by w_g^{t k}: tw[(t k L / g)] (conjugated for sign > 0).
+template <int R>
+__device__ inline void butterfly(float2 *buf, int base, int qstride, const float2 *tw, int twstep, int t, float sign)
+{
+    float2 v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        float2 a = buf[base + k * qstride];
+        if (k > 0 && t > 0) {
+            float2 w = tw[t * k * twstep];
+            w.y *= -sign; // table holds e^{-2 pi i k/L}; sign -1 keeps it, +1 conjugates
+            a = cmul(a, w);
+        }
+        v[k] = a;
+    }
+    if (R == 2) {
+        buf[base] = cadd(v[0], v[1]);
+        buf[base + qstride] = csub(v[0], v[1]);
+    } else if (R == 4) {
+        const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const float2 s13 = cadd(v[1], v[3]), d13 = mul_si(csub(v[1], v[3]), sign);
+        buf[base] = cadd(s02, s13);
+        buf[base + qstride] = cadd(d02, d13);
+        buf[base + 2 * qstride] = csub(s02, s13);
+        buf[base + 3 * qstride] = csub(d02, d13);
+    } else if (R == 3) {
+        const float c1 = -0.5f, s1 = 0.86602540378443864676f * sign;
+        const float2 a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
+        buf[base] = cadd(v[0], a);
+        const float2 m = make_float2(v[0].x + c1 * a.x, v[0].y + c1 * a.y);
+        const float2 r = make_float2(-s1 * b.y, s1 * b.x);
+        buf[base + qstride] = cadd(m, r);
+        buf[base + 2 * qstride] = csub(m, r);
+    } else if (R == 5) {
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f * sign, s2 = 0.58778525229247312917f * sign;
+        const float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        const float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        buf[base] = cadd(v[0], cadd(a1, a2));
+        const float2 m1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+        const float2 m2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+        // r1 = i (s1 b1 + s2 b2), r2 = i (s2 b1 - s1 b2)
+        const float2 r1 = make_float2(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
+        const float2 r2 = make_float2(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
+        buf[base + qstride] = cadd(m1, r1);
+        buf[base + 4 * qstride] = csub(m1, r1);
+        buf[base + 2 * qstride] = cadd(m2, r2);
+        buf[base + 3 * qstride] = csub(m2, r2);
+    }
+}
+
+// floor(n / d) for 0 <= n < 2^22, 1 <= d <= 2^12, with r = 1/d: the float estimate
+// is within one of the quotient, and one correction step each way makes it exact.
+__device__ inline int fdiv(int n, int d, float r)
+{
+    int q = (int)((float)n * r);
+    q -= (q * d > n) ? 1 : 0;
+    q += ((q + 1) * d <= n) ? 1 : 0;
+    return q;
+}
+
+// In-place FFT of `nlines` lines of length L held in LDS: element i of line j at
+// buf[j * lstride + i * estride].  All threads of the block take part.
+__device__ inline void fft_lines(float2 *buf, int nlines, int lstride, int estride, int L, const Stages &st,
+                                 const float2 *tw, float sign)
+{
+    const float rl = 1.0f / (float)nlines;
+    int q = 1;
+    for (int s = 0; s < st.n; ++s) {
+        const int r = st.radix[s];
+        const int g = r * q;
+        const int twstep = L / g;
+        const int nb = L / r; // butterflies per line
+        const float rq = 1.0f / (float)q;
+        for (int w = threadIdx.x; w < nb * nlines; w += blockDim.x) {
+            // lines fastest: neighbouring threads work on neighbouring lines
+            const int bidx = fdiv(w, nlines, rl);
+            const int j = w - bidx * nlines;
+            const int blk = fdiv(bidx, q, rq);
+            const int t = bidx - blk * q;
+            const int base = j * lstride + (blk * g + t) * estride;
+            switch (r) {
+            case 2: butterfly<2>(buf, base, q * estride, tw, twstep, t, sign); break;
+            case 3: butterfly<3>(buf, base, q * estride, tw, twstep, t, sign); break;
+            case 4: butterfly<4>(buf, base, q * estride, tw, twstep, t, sign); break;
+            default: butterfly<5>(buf, base, q * estride, tw, twstep, t, sign); break;
+            }
+        }
+        __syncthreads();
+        q = g;
+    }
+}
+
+__device__ inline void load_table(float2 *dst, const float2 *src, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Column pass: FFT along a strided axis.  A block takes CB consecutive columns of
+// one slab; rows are the transformed axis.  Input row n sits at FFT position
+// (n - m/2) mod L (in_map) or n; output row n reads FFT index (n - m/2) mod L
+// (out_map) or n.
+// ---------------------------------------------------------------------------
+
+struct ColArgs {
+    float2 *data; // in place
+    int L;
+    int rows_in, rows_out;
+    int in_map, out_map; // 1: pruned position map with m = rows_in (in) / rows_out (out)
+    int64_t ncols;       // contiguous columns; also the row stride
+    int64_t slab_inner;  // slab s starts at (s / slab_inner) * stride_outer + (s % slab_inner) * stride_inner
+    int64_t stride_outer, stride_inner;
+    int CB;
+    float sign;
+    Stages st;
+    const float2 *tw;
+    const int *pinv; // FFT position -> LDS slot (digit reversal)
+};
+
+__global__ __launch_bounds__(kFFTThreads) void k_fft_cols(ColArgs a)
+{
+    extern __shared__ float2 lds_fft[];
+    float2 *tw = lds_fft;
+    float2 *buf = lds_fft + a.L;
+    const int CB = a.CB;
+    const int lstride = 1;        // columns adjacent
+    const int estride = CB + 1;   // rows padded by one element
+    const int64_t col0 = (int64_t)blockIdx.x * CB;
+    const int64_t slab = blockIdx.y;
+    const int ncb = (int)min((int64_t)CB, a.ncols - col0);
+    float2 *base = a.data + (slab / a.slab_inner) * a.stride_outer + (slab % a.slab_inner) * a.stride_inner + col0;
+    load_table(tw, a.tw, a.L);
+    if (a.rows_in < a.L) // pruned input: zero every slot first
+        for (int e = threadIdx.x; e < a.L * estride; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int half_in = a.rows_in / 2;
+    const float rcb = 1.0f / (float)CB;
+    for (int e = threadIdx.x; e < a.rows_in * CB; e += blockDim.x) {
+        const int n = fdiv(e, CB, rcb), j = e - n * CB;
+        if (j >= ncb) continue;
+        int p = n;
+        if (a.in_map) {
+            p = n - half_in;
+            if (p < 0) p += a.L;
+        }
+        buf[a.pinv[p] * estride + j] = base[(int64_t)n * a.ncols + j];
+    }
+    __syncthreads();
+    fft_lines(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
+    const int half_out = a.rows_out / 2;
+    for (int e = threadIdx.x; e < a.rows_out * CB; e += blockDim.x) {
+        const int n = fdiv(e, CB, rcb), j = e - n * CB;
+        if (j >= ncb) continue;
+        int p = n;
+        if (a.out_map) {
+            p = n - half_out;
+            if (p < 0) p += a.L;
+        }
+        base[(int64_t)n * a.ncols + j] = buf[p * estride + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row passes along the contiguous image axis, fused with the image read (type 2)
+// or write (type 1): deapodization, scale, zero-pad / crop, coil transpose.  A
+// block takes one image line position (all leading indices but the last axis)
+// and a chunk of CL coils; LDS holds CL lines of L (+1) elements.
+// ---------------------------------------------------------------------------
+
+struct RowArgs {
+    const float2 *img_in; // type 2: image
+    float2 *img_out;      // type 1: image
+    float2 *grid;         // line (c, n0, n1) at c * G + n0 * s0 + n1 * L
+    int64_t G, s0;
+    int L;
+    int m;                // image points on this axis
+    int64_t nlines;       // image lines per coil (product of the other modes)
+    int64_t nc;
+    int CL;               // coils per block
+    int layout;
+    float scale;
+    float sign;
+    const float *deap_last; // deapodization along this axis (m)
+    const float *deap_a;    // along axis 0 (m0 entries)
+    const float *deap_b;    // along axis 1 (3-D only; m1 entries), null in 2-D
+    int m_b;                // m1 in 3-D (line index = n0 * m1 + n1), 1 in 2-D
+    Stages st;
+    const float2 *tw;
+    const int *pinv;
+};
+
+__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a)
+{
+    extern __shared__ float2 lds_fft[];
+    float2 *tw = lds_fft;
+    float2 *buf = lds_fft + a.L;
+    const int64_t line = blockIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.y * a.CL;
+    const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
+    const int ls = a.L + 1;
+    load_table(tw, a.tw, a.L);
+    for (int e = threadIdx.x; e < ncl * ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int64_t n0 = line / a.m_b, n1 = line - (line / a.m_b) * a.m_b;
+    const float d01 = a.scale * a.deap_a[n0] * (a.deap_b ? a.deap_b[n1] : 1.0f);
+    const int half = a.m / 2;
+    const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
+    for (int e = threadIdx.x; e < a.m * ncl; e += blockDim.x) {
+        int n, c;
+        float2 v;
+        if (a.layout == GN_COIL_LAST) {
+            n = fdiv(e, ncl, rncl);
+            c = e - n * ncl;
+            v = a.img_in[(line * a.m + n) * a.nc + c0 + c];
+        } else {
+            c = fdiv(e, a.m, rm);
+            n = e - c * a.m;
+            v = a.img_in[(c0 + c) * a.nlines * a.m + line * a.m + n];
+        }
+        const float d = d01 * a.deap_last[n];
+        int p = n - half;
+        if (p < 0) p += a.L;
+        buf[c * ls + a.pinv[p]] = make_float2(v.x * d, v.y * d);
+    }
+    __syncthreads();
+    fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
+    const int64_t off = n0 * a.s0 + n1 * a.L;
+    for (int e = threadIdx.x; e < ncl * a.L; e += blockDim.x) {
+        const int c = fdiv(e, a.L, rL), k = e - c * a.L;
+        a.grid[(c0 + c) * a.G + off + k] = buf[c * ls + k];
+    }
+}
+
+__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop(RowArgs a)
+{
+    extern __shared__ float2 lds_fft[];
+    float2 *tw = lds_fft;
+    float2 *buf = lds_fft + a.L;
+    const int64_t line = blockIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.y * a.CL;
+    const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
+    const int ls = a.L + 1;
+    const int64_t n0 = line / a.m_b, n1 = line - (line / a.m_b) * a.m_b;
+    const int64_t off = n0 * a.s0 + n1 * a.L;
+    const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
+    load_table(tw, a.tw, a.L);
+    for (int e = threadIdx.x; e < ncl * a.L; e += blockDim.x) {
+        const int c = fdiv(e, a.L, rL), k = e - c * a.L;
+        buf[c * ls + a.pinv[k]] = a.grid[(c0 + c) * a.G + off + k];
+    }
+    __syncthreads();
+    fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
+    const float d01 = a.scale * a.deap_a[n0] * (a.deap_b ? a.deap_b[n1] : 1.0f);
+    const int half = a.m / 2;
+    for (int e = threadIdx.x; e < a.m * ncl; e += blockDim.x) {
+        int n, c;
+        if (a.layout == GN_COIL_LAST) {
+            n = fdiv(e, ncl, rncl);
+            c = e - n * ncl;
+        } else {
+            c = fdiv(e, a.m, rm);
+            n = e - c * a.m;
+        }
+        int p = n - half;
+        if (p < 0) p += a.L;
+        const float d = d01 * a.deap_last[n];
+        const float2 v = buf[c * ls + p];
+        const float2 o = make_float2(v.x * d, v.y * d);
+        if (a.layout == GN_COIL_LAST)
+            a.img_out[(line * a.m + n) * a.nc + c0 + c] = o;
+        else
+            a.img_out[(c0 + c) * a.nlines * a.m + line * a.m + n] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+std::vector<int> fft_radices(int L)
+{
+    std::vector<int> r;
+    int n = L;
+    while (n % 4 == 0 && n > 1) {
+        r.push_back(4);
+        n /= 4;
+    }
+    while (n % 2 == 0 && n > 1) {
+        r.push_back(2);
+        n /= 2;
+    }
+    for (int p : {3, 5})
+        while (n % p == 0 && n > 1) {
+            r.push_back(p);
+            n /= p;
+        }
+    if (n != 1) return {};
+    std::reverse(r.begin(), r.end());
+    return r;
+}
+
+Stages make_stages(const FFTAxis &ax)
+{
+    Stages s;
+    s.n = ax.nst;
+    for (int i = 0; i < kFFTMaxStages; ++i) s.radix[i] = i < ax.nst ? ax.radix[i] : 1;
+    return s;
+}
+
+size_t col_lds(int L, int CB) { return (size_t)L * sizeof(float2) + (size_t)L * (CB + 1) * sizeof(float2); }
+size_t row_lds(int L, int CL) { return (size_t)L * sizeof(float2) + (size_t)CL * (L + 1) * sizeof(float2); }
+
+int pick_cb(int L)
+{
+    int cb = 16;
+    while (cb > 1 && col_lds(L, cb) > 64 * 1024) cb /= 2;
+    return cb;
+}
+
+int pick_cl(int L, int64_t nc)
+{
+    int cl = (int)std::min<int64_t>(nc, 16);
+    while (cl > 1 && row_lds(L, cl) > 64 * 1024) --cl;
+    return cl;
+}
+
+hipError_t col_pass(const FFTAxis &ax, float sign, float2 *data, int rows_in, int rows_out, int in_map, int out_map,
+                    int64_t ncols, int64_t nslabs, int64_t slab_inner, int64_t stride_outer, int64_t stride_inner,
+                    hipStream_t st)
+{
+    ColArgs a;
+    a.data = data;
+    a.L = ax.L;
+    a.rows_in = rows_in;
+    a.rows_out = rows_out;
+    a.in_map = in_map;
+    a.out_map = out_map;
+    a.ncols = ncols;
+    a.slab_inner = slab_inner;
+    a.stride_outer = stride_outer;
+    a.stride_inner = stride_inner;
+    a.CB = pick_cb(ax.L);
+    a.sign = sign;
+    a.st = make_stages(ax);
+    a.tw = ax.tw;
+    a.pinv = ax.pinv;
+    dim3 grid((unsigned)((ncols + a.CB - 1) / a.CB), (unsigned)nslabs);
+    hipLaunchKernelGGL(k_fft_cols, grid, dim3(kFFTThreads), col_lds(ax.L, a.CB), st, a);
+    return hipGetLastError();
+}
+
+RowArgs row_args(const gn_plan_s *p, const FFTAxis &ax, float sign, float scale)
+{
+    const Geometry &g = p->geo;
+    const int d = g.dim;
+    RowArgs a;
+    a.img_in = nullptr;
+    a.img_out = nullptr;
+    a.grid = p->grid;
+    a.G = g.G;
+    a.s0 = g.G / g.nf[0]; // nf1 (* nf2)
+    a.L = ax.L;
+    a.m = (int)g.modes[d - 1];
+    a.nlines = g.N / g.modes[d - 1];
+    a.nc = p->nc;
+    a.CL = pick_cl(ax.L, p->nc);
+    a.layout = p->layout;
+    a.scale = scale;
+    a.sign = sign;
+    a.deap_a = p->deap;
+    if (d == 3) {
+        a.deap_b = p->deap + g.modes[0];
+        a.deap_last = p->deap + g.modes[0] + g.modes[1];
+        a.m_b = (int)g.modes[1];
+    } else {
+        a.deap_b = nullptr;
+        a.deap_last = p->deap + g.modes[0];
+        a.m_b = 1;
+    }
+    a.st = make_stages(ax);
+    a.tw = ax.tw;
+    a.pinv = ax.pinv;
+    return a;
+}
+
+} // namespace
+
+bool fft_length_supported(int64_t L)
+{
+    if (L < 2 || L > 2048) return false; // LDS: one line of 2048 + twiddles fits in 64 KiB
+    return !fft_radices((int)L).empty() && (int)fft_radices((int)L).size() <= kFFTMaxStages;
+}
+
+int fft_axis_init(FFTAxis *ax, int L)
+{
+    std::vector<int> r = fft_radices(L);
+    if (r.empty() || (int)r.size() > kFFTMaxStages) return fail(GN_EINVAL, "FFT length not supported");
+    ax->L = L;
+    ax->nst = (int)r.size();
+    for (int i = 0; i < ax->nst; ++i) ax->radix[i] = r[i];
+    // digit reversal: slot i holds position rev(i); pinv maps position -> slot
+    std::vector<int> pinv(L);
+    for (int i = 0; i < L; ++i) {
+        int rem = i, rev = 0, scale = L;
+        for (int rr : r) {
+            const int dgt = rem % rr;
+            rem /= rr;
+            scale /= rr;
+            rev += dgt * scale;
+        }
+        pinv[rev] = i;
+    }
+    std::vector<float2> tw(L);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int k = 0; k < L; ++k) {
+        const double ang = -two_pi * (double)k / (double)L;
+        tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    if (hipMalloc((void **)&ax->tw, L * sizeof(float2)) != hipSuccess ||
+        hipMalloc((void **)&ax->pinv, L * sizeof(int)) != hipSuccess)
+        return fail(GN_ENOMEM, "FFT tables");
+    hipError_t e = hipMemcpy(ax->tw, tw.data(), L * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ax->pinv, pinv.data(), L * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "FFT tables");
+    return GN_OK;
+}
+
+void fft_axis_free(FFTAxis *ax)
+{
+    hipFree(ax->tw);
+    hipFree(ax->pinv);
+    ax->tw = nullptr;
+    ax->pinv = nullptr;
+}
+
+// Type 2, pass A: image -> leading grid rows (pad, deapodize, scale, FFT along the last axis).
+int pruned_type2_rows(gn_plan_s *p, const float2 *image, float scale, float sign)
+{
+    const Geometry &g = p->geo;
+    const FFTAxis &az = p->fft_ax[g.dim - 1];
+    RowArgs ra = row_args(p, az, sign, scale);
+    ra.img_in = image;
+    dim3 ga((unsigned)ra.nlines, (unsigned)((p->nc + ra.CL - 1) / ra.CL));
+    hipLaunchKernelGGL(k_fft_rows_pad, ga, dim3(kFFTThreads), row_lds(az.L, ra.CL), p->stream, ra);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GN_OK : hip_fail(e, "FFT pass A");
+}
+
+// Type 2, passes B (3-D) and C: the remaining axes, expanding m rows to nf.
+int pruned_type2_cols(gn_plan_s *p, float sign)
+{
+    const Geometry &g = p->geo;
+    const int64_t nc = p->nc, m0 = g.modes[0], L2 = g.nf[g.dim - 1];
+    hipError_t e;
+    if (g.dim == 3) {
+        // B: slabs (coil, n0), rows n1 (m1 -> nf1), columns k2
+        e = col_pass(p->fft_ax[1], sign, p->grid, (int)g.modes[1], (int)g.nf[1], 1, 0, L2, nc * m0, m0, g.G,
+                     g.nf[1] * L2, p->stream);
+        if (e != hipSuccess) return hip_fail(e, "FFT pass B");
+    }
+    // C: slabs = coils, rows n0 (m0 -> nf0), columns = all cells of the other axes
+    e = col_pass(p->fft_ax[0], sign, p->grid, (int)m0, (int)g.nf[0], 1, 0, g.G / g.nf[0], nc, 1, g.G, 0, p->stream);
+    return e == hipSuccess ? GN_OK : hip_fail(e, "FFT pass C");
+}
+
+// Type 1, passes C' and B' (3-D): keep m of nf rows on the leading axes (in place).
+int pruned_type1_cols(gn_plan_s *p, float sign)
+{
+    const Geometry &g = p->geo;
+    const int64_t nc = p->nc, m0 = g.modes[0], L2 = g.nf[g.dim - 1];
+    hipError_t e = col_pass(p->fft_ax[0], sign, p->grid, (int)g.nf[0], (int)m0, 0, 1, g.G / g.nf[0], nc, 1, g.G, 0,
+                            p->stream);
+    if (e != hipSuccess) return hip_fail(e, "FFT pass C'");
+    if (g.dim == 3) {
+        e = col_pass(p->fft_ax[1], sign, p->grid, (int)g.nf[1], (int)g.modes[1], 0, 1, L2, nc * m0, m0, g.G,
+                     g.nf[1] * L2, p->stream);
+        if (e != hipSuccess) return hip_fail(e, "FFT pass B'");
+    }
+    return GN_OK;
+}
+
+// Type 1, pass A': FFT along the last axis, crop, deapodize, scale -> image.
+int pruned_type1_rows(gn_plan_s *p, float2 *image, float scale, float sign)
+{
+    const Geometry &g = p->geo;
+    const FFTAxis &az = p->fft_ax[g.dim - 1];
+    RowArgs ra = row_args(p, az, sign, scale);
+    ra.img_out = image;
+    dim3 ga((unsigned)ra.nlines, (unsigned)((p->nc + ra.CL - 1) / ra.CL));
+    hipLaunchKernelGGL(k_fft_rows_crop, ga, dim3(kFFTThreads), row_lds(az.L, ra.CL), p->stream, ra);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GN_OK : hip_fail(e, "FFT pass A'");
+}
+
+} // namespace gn

@@ -17,6 +17,17 @@ constexpr int kGatherThreads = 256; // threads per gather workgroup
 constexpr int kCoilGroup = 8;       // coils accumulated together per pass
 constexpr int kSpreadChunk = 1024;  // max (tile, sample) pairs per spread work item
 constexpr int kGatherChunk = 1024;  // max samples per gather work item
+constexpr int kFFTMaxStages = 12;   // radix stages of one pruned-FFT axis
+
+// One axis of the in-LDS mixed-radix FFT: radix sequence (decimation in time),
+// twiddle table e^{-2 pi i k/L} and the digit-reversal map position -> LDS slot.
+struct FFTAxis {
+    int L = 0;
+    int nst = 0;
+    int radix[kFFTMaxStages] = {0};
+    float2 *tw = nullptr;
+    int *pinv = nullptr;
+};
 
 // Sample records in tile-sorted order: start cell of the window on each axis
 // (already wrapped to [0,nf)), the sample's original index, its offset
@@ -144,6 +155,11 @@ struct gn_plan_s {
     float *deap = nullptr;    // concatenated per-axis exp(tau k^2) tables
     int *err_flag = nullptr;  // device flag set by set_points on bad coordinates
 
+    // FFT engine: the pruned in-place passes (gn_fft.hip) when every nf is a product
+    // of 2, 3, 5 (fft_custom), else rocFFT; GN_FFT_ROCFFT forces rocFFT
+    bool fft_custom = false;
+    int fft_engine = GN_FFT_AUTO;
+    gn::FFTAxis fft_ax[3];
     rocfft_plan fft_fwd = nullptr;
     rocfft_plan fft_bwd = nullptr;
     rocfft_execution_info fft_info = nullptr;
@@ -177,6 +193,15 @@ int launch_pad(gn_plan_s *p, const float2 *image, float scale);
 int launch_crop(gn_plan_s *p, float2 *image, float scale);
 int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out);
 void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile);
+
+// fft.hip
+bool fft_length_supported(int64_t L);
+int fft_axis_init(FFTAxis *ax, int L);
+void fft_axis_free(FFTAxis *ax);
+int pruned_type2_rows(gn_plan_s *p, const float2 *image, float scale, float sign);
+int pruned_type2_cols(gn_plan_s *p, float sign);
+int pruned_type1_cols(gn_plan_s *p, float sign);
+int pruned_type1_rows(gn_plan_s *p, float2 *image, float scale, float sign);
 
 // api.hip
 int plan_create_impl(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,

@@ -223,6 +223,12 @@ class Plan:
             check(_lib.load().gn_plan_sample_order(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return out
 
+    def set_fft_engine(self, engine="auto"):
+        """"auto": pruned FFT passes fused with pad/crop where every nf is a product of
+        2, 3 and 5; "rocfft": separate pad, rocFFT, crop."""
+        check(_lib.load().gn_plan_set_fft_engine(self._h, _lib.GN_FFT_ROCFFT if engine == "rocfft"
+                                                 else _lib.GN_FFT_AUTO))
+
     def set_timing(self, enable=True):
         """Enable per-stage HIP event timers inside gn_execute."""
         check(_lib.load().gn_plan_set_timing(self._h, 1 if enable else 0))

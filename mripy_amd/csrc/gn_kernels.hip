@@ -24,6 +24,24 @@ namespace gn {
 namespace {
 
 constexpr double kTwoPi = 6.283185307179586; // 2*np.pi in float64
+constexpr int kXCDs = 8;                      // MI355X: 8 XCDs, each with its own L2
+
+// Workgroups are dispatched to the XCDs round-robin by linear block id.  Give each
+// XCD runs of kXCDRun consecutive work items (neighbouring tiles: shared halo cells
+// and sample values stay in one L2) while the runs still rotate over the XCDs, so a
+// dense region of the trajectory is shared out evenly.  Blocks past the last whole
+// round of runs keep their order; the map is a bijection on [0, n).
+#ifndef GN_XCD_RUN
+#define GN_XCD_RUN 32
+#endif
+constexpr int64_t kXCDRun = GN_XCD_RUN;
+__device__ inline int64_t xcd_swizzle(int64_t lin, int64_t n)
+{
+    const int64_t round = kXCDs * kXCDRun;
+    if (lin >= (n / round) * round) return lin;
+    const int64_t x = lin % kXCDs, k = lin / kXCDs;
+    return ((k / kXCDRun) * kXCDs + x) * kXCDRun + k % kXCDRun;
+}
 
 // ---------------------------------------------------------------------------
 // set_points: window start cell, offset and home tile of every sample (FP64)
@@ -604,7 +622,12 @@ __device__ inline void spread_body(const SpreadArgs &a, const int4 item, const i
 template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
 {
-    spread_body<D, NCG>(a, a.items[blockIdx.x], blockIdx.y, blockIdx.z);
+    const int64_t n = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t lin = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+    const int64_t w = xcd_swizzle(lin, n);
+    const int bx = (int)(w % gridDim.x);
+    const int64_t yz = w / gridDim.x;
+    spread_body<D, NCG>(a, a.items[bx], (int)(yz % gridDim.y), (int)(yz / gridDim.y));
 }
 
 // Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0}.
@@ -797,7 +820,9 @@ __device__ inline void gather_body(const GatherArgs &a, const int4 item, const i
 template <int D, int NS, bool WRAP>
 __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
 {
-    gather_body<D, NS, WRAP>(a, a.items[blockIdx.x], blockIdx.y);
+    const int64_t n = (int64_t)gridDim.x * gridDim.y;
+    const int64_t w = xcd_swizzle(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, n);
+    gather_body<D, NS, WRAP>(a, a.items[w % gridDim.x], (int)(w / gridDim.x));
 }
 
 // Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0}.

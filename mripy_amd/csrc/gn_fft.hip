@@ -32,7 +32,17 @@
 namespace gn {
 namespace {
 
-constexpr int kFFTThreads = 256;
+#ifndef GN_FFT_THREADS
+#define GN_FFT_THREADS 256
+#endif
+#ifndef GN_FFT_COL_THREADS
+#define GN_FFT_COL_THREADS 512
+#endif
+#ifndef GN_FFT_CB
+#define GN_FFT_CB 16
+#endif
+constexpr int kFFTThreads = GN_FFT_THREADS;       // row passes
+constexpr int kFFTColThreads = GN_FFT_COL_THREADS; // column passes
 
 struct Stages {
     int n;
@@ -166,7 +176,7 @@ struct ColArgs {
     const int *pinv; // FFT position -> LDS slot (digit reversal)
 };
 
-__global__ __launch_bounds__(kFFTThreads) void k_fft_cols(ColArgs a)
+__global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
 {
     extern __shared__ float2 lds_fft[];
     float2 *tw = lds_fft;
@@ -178,7 +188,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(ColArgs a)
     const int64_t slab = blockIdx.y;
     const int ncb = (int)min((int64_t)CB, a.ncols - col0);
     float2 *base = a.data + (slab / a.slab_inner) * a.stride_outer + (slab % a.slab_inner) * a.stride_inner + col0;
+    int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
     load_table(tw, a.tw, a.L);
+    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     if (a.rows_in < a.L) // pruned input: zero every slot first
         for (int e = threadIdx.x; e < a.L * estride; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
@@ -192,7 +204,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_cols(ColArgs a)
             p = n - half_in;
             if (p < 0) p += a.L;
         }
-        buf[a.pinv[p] * estride + j] = base[(int64_t)n * a.ncols + j];
+        buf[spinv[p] * estride + j] = base[(int64_t)n * a.ncols + j];
     }
     __syncthreads();
     fft_lines(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
@@ -247,7 +259,9 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a)
     const int64_t c0 = (int64_t)blockIdx.y * a.CL;
     const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
     const int ls = a.L + 1;
+    int *spinv = (int *)(buf + a.CL * ls);
     load_table(tw, a.tw, a.L);
+    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     for (int e = threadIdx.x; e < ncl * ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
     const int64_t n0 = line / a.m_b, n1 = line - (line / a.m_b) * a.m_b;
@@ -269,7 +283,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a)
         const float d = d01 * a.deap_last[n];
         int p = n - half;
         if (p < 0) p += a.L;
-        buf[c * ls + a.pinv[p]] = make_float2(v.x * d, v.y * d);
+        buf[c * ls + spinv[p]] = make_float2(v.x * d, v.y * d);
     }
     __syncthreads();
     fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
@@ -292,10 +306,13 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop(RowArgs a)
     const int64_t n0 = line / a.m_b, n1 = line - (line / a.m_b) * a.m_b;
     const int64_t off = n0 * a.s0 + n1 * a.L;
     const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
+    int *spinv = (int *)(buf + a.CL * ls);
     load_table(tw, a.tw, a.L);
+    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
+    __syncthreads();
     for (int e = threadIdx.x; e < ncl * a.L; e += blockDim.x) {
         const int c = fdiv(e, a.L, rL), k = e - c * a.L;
-        buf[c * ls + a.pinv[k]] = a.grid[(c0 + c) * a.G + off + k];
+        buf[c * ls + spinv[k]] = a.grid[(c0 + c) * a.G + off + k];
     }
     __syncthreads();
     fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
@@ -356,12 +373,19 @@ Stages make_stages(const FFTAxis &ax)
     return s;
 }
 
-size_t col_lds(int L, int CB) { return (size_t)L * sizeof(float2) + (size_t)L * (CB + 1) * sizeof(float2); }
-size_t row_lds(int L, int CL) { return (size_t)L * sizeof(float2) + (size_t)CL * (L + 1) * sizeof(float2); }
+// LDS: twiddles | lines | digit-reversal map
+size_t col_lds(int L, int CB)
+{
+    return (size_t)L * sizeof(float2) + (size_t)L * (CB + 1) * sizeof(float2) + (size_t)L * sizeof(int);
+}
+size_t row_lds(int L, int CL)
+{
+    return (size_t)L * sizeof(float2) + (size_t)CL * (L + 1) * sizeof(float2) + (size_t)L * sizeof(int);
+}
 
 int pick_cb(int L)
 {
-    int cb = 16;
+    int cb = GN_FFT_CB;
     while (cb > 1 && col_lds(L, cb) > 64 * 1024) cb /= 2;
     return cb;
 }
@@ -394,7 +418,7 @@ hipError_t col_pass(const FFTAxis &ax, float sign, float2 *data, int rows_in, in
     a.tw = ax.tw;
     a.pinv = ax.pinv;
     dim3 grid((unsigned)((ncols + a.CB - 1) / a.CB), (unsigned)nslabs);
-    hipLaunchKernelGGL(k_fft_cols, grid, dim3(kFFTThreads), col_lds(ax.L, a.CB), st, a);
+    hipLaunchKernelGGL(k_fft_cols, grid, dim3(kFFTColThreads), col_lds(ax.L, a.CB), st, a);
     return hipGetLastError();
 }
 

@@ -13,10 +13,19 @@
 namespace gn {
 
 constexpr int kMaxDim = 3;
-constexpr int kGatherThreads = 256; // threads per gather workgroup
+#ifndef GN_GATHER_THREADS
+#define GN_GATHER_THREADS 512
+#endif
+#ifndef GN_GATHER_CHUNK
+#define GN_GATHER_CHUNK 256
+#endif
+constexpr int kGatherThreads = GN_GATHER_THREADS; // threads per gather workgroup
 constexpr int kCoilGroup = 8;       // coils accumulated together per pass
-constexpr int kSpreadChunk = 1024;  // max (tile, sample) pairs per spread work item
-constexpr int kGatherChunk = 1024;  // max samples per gather work item
+#ifndef GN_SPREAD_CHUNK
+#define GN_SPREAD_CHUNK 1024
+#endif
+constexpr int kSpreadChunk = GN_SPREAD_CHUNK; // max (tile, sample) pairs per spread work item
+constexpr int kGatherChunk = GN_GATHER_CHUNK;  // max samples per gather work item
 constexpr int kFFTMaxStages = 12;   // radix stages of one pruned-FFT axis
 
 // One axis of the in-LDS mixed-radix FFT: radix sequence (decimation in time),

@@ -42,6 +42,7 @@ namespace {
 #define GN_FFT_CB 16
 #endif
 constexpr int kFFTThreads = GN_FFT_THREADS;       // row passes
+constexpr int kLoadUnroll = 8;                    // global loads in flight per thread
 constexpr int kFFTColThreads = GN_FFT_COL_THREADS; // column passes
 
 struct Stages {
@@ -196,15 +197,29 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
     __syncthreads();
     const int half_in = a.rows_in / 2;
     const float rcb = 1.0f / (float)CB;
-    for (int e = threadIdx.x; e < a.rows_in * CB; e += blockDim.x) {
-        const int n = fdiv(e, CB, rcb), j = e - n * CB;
-        if (j >= ncb) continue;
-        int p = n;
-        if (a.in_map) {
-            p = n - half_in;
-            if (p < 0) p += a.L;
+    const int total_in = a.rows_in * CB;
+    // kLoadUnroll independent global loads in flight per thread before the LDS stores
+    for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
+        float2 v[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int n = fdiv(e, CB, rcb), j = e - n * CB;
+            if (e < total_in && j < ncb) v[u] = base[(int64_t)n * a.ncols + j];
         }
-        buf[spinv[p] * estride + j] = base[(int64_t)n * a.ncols + j];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int n = fdiv(e, CB, rcb), j = e - n * CB;
+            if (e < total_in && j < ncb) {
+                int p = n;
+                if (a.in_map) {
+                    p = n - half_in;
+                    if (p < 0) p += a.L;
+                }
+                buf[spinv[p] * estride + j] = v[u];
+            }
+        }
     }
     __syncthreads();
     fft_lines(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
@@ -268,22 +283,37 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a)
     const float d01 = a.scale * a.deap_a[n0] * (a.deap_b ? a.deap_b[n1] : 1.0f);
     const int half = a.m / 2;
     const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
-    for (int e = threadIdx.x; e < a.m * ncl; e += blockDim.x) {
-        int n, c;
-        float2 v;
-        if (a.layout == GN_COIL_LAST) {
-            n = fdiv(e, ncl, rncl);
-            c = e - n * ncl;
-            v = a.img_in[(line * a.m + n) * a.nc + c0 + c];
-        } else {
-            c = fdiv(e, a.m, rm);
-            n = e - c * a.m;
-            v = a.img_in[(c0 + c) * a.nlines * a.m + line * a.m + n];
+    const int total = a.m * ncl;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kLoadUnroll * blockDim.x) {
+        float2 v[kLoadUnroll];
+        int nn[kLoadUnroll], cc[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const int e = e0 + u * blockDim.x;
+            int n, c;
+            if (a.layout == GN_COIL_LAST) {
+                n = fdiv(e, ncl, rncl);
+                c = e - n * ncl;
+            } else {
+                c = fdiv(e, a.m, rm);
+                n = e - c * a.m;
+            }
+            nn[u] = n;
+            cc[u] = c;
+            if (e < total)
+                v[u] = a.layout == GN_COIL_LAST ? a.img_in[(line * a.m + n) * a.nc + c0 + c]
+                                                : a.img_in[(c0 + c) * a.nlines * a.m + line * a.m + n];
         }
-        const float d = d01 * a.deap_last[n];
-        int p = n - half;
-        if (p < 0) p += a.L;
-        buf[c * ls + spinv[p]] = make_float2(v.x * d, v.y * d);
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < total) {
+                const float d = d01 * a.deap_last[nn[u]];
+                int p = nn[u] - half;
+                if (p < 0) p += a.L;
+                buf[cc[u] * ls + spinv[p]] = make_float2(v[u].x * d, v[u].y * d);
+            }
+        }
     }
     __syncthreads();
     fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
@@ -310,9 +340,21 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop(RowArgs a)
     load_table(tw, a.tw, a.L);
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     __syncthreads();
-    for (int e = threadIdx.x; e < ncl * a.L; e += blockDim.x) {
-        const int c = fdiv(e, a.L, rL), k = e - c * a.L;
-        buf[c * ls + spinv[k]] = a.grid[(c0 + c) * a.G + off + k];
+    const int total = ncl * a.L;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kLoadUnroll * blockDim.x) {
+        float2 v[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int c = fdiv(e, a.L, rL), k = e - c * a.L;
+            if (e < total) v[u] = a.grid[(c0 + c) * a.G + off + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int c = fdiv(e, a.L, rL), k = e - c * a.L;
+            if (e < total) buf[c * ls + spinv[k]] = v[u];
+        }
     }
     __syncthreads();
     fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);

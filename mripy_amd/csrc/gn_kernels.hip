@@ -35,6 +35,58 @@ constexpr int kXCDs = 8;                      // MI355X: 8 XCDs, each with its o
 #define GN_XCD_RUN 32
 #endif
 constexpr int64_t kXCDRun = GN_XCD_RUN;
+// floor(n / d) for 0 <= n < 2^16 (or n < 2^22 with d <= 2^12) given r = 1/d: the
+// float estimate is within one of the quotient and one correction step each way
+// makes it exact (checked exhaustively over those ranges).
+__device__ inline int fdiv_small(int n, int d, float r)
+{
+    int q = (int)((float)n * r);
+    q -= (q * d > n) ? 1 : 0;
+    q += ((q + 1) * d <= n) ? 1 : 0;
+    return q;
+}
+
+constexpr int kStageUnroll = 8; // global loads in flight per thread while staging
+
+// Stage the (R0, R1, R2) region at org (wrapped) of one coil's grid g into LDS:
+// kStageUnroll independent loads per thread are issued before any LDS store.
+template <int D>
+__device__ inline void stage_region(float2 *lds, const float2 *__restrict__ g, const int64_t *nf, const int *org,
+                                    int R1, int R2, int RC)
+{
+    const int R12 = R1 * R2;
+    const float rR12 = 1.0f / (float)R12, rR2 = 1.0f / (float)R2;
+    for (int base = threadIdx.x; base < RC; base += kStageUnroll * blockDim.x) {
+        float2 v[kStageUnroll];
+#pragma unroll
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int rr = base + u * blockDim.x;
+            if (rr < RC) {
+                const int r0 = fdiv_small(rr, R12, rR12);
+                const int rem = rr - r0 * R12;
+                const int r1 = D == 3 ? fdiv_small(rem, R2, rR2) : rem;
+                const int r2 = D == 3 ? rem - r1 * R2 : 0;
+                int q0 = org[0] + r0;
+                if (q0 >= nf[0]) q0 -= (int)nf[0];
+                int q1 = org[1] + r1;
+                if (q1 >= nf[1]) q1 -= (int)nf[1];
+                int64_t gi = (int64_t)q0 * nf[1] + q1;
+                if (D == 3) {
+                    int q2 = org[2] + r2;
+                    if (q2 >= nf[2]) q2 -= (int)nf[2];
+                    gi = gi * nf[2] + q2;
+                }
+                v[u] = g[gi];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int rr = base + u * blockDim.x;
+            if (rr < RC) lds[rr] = v[u];
+        }
+    }
+}
+
 __device__ inline int64_t xcd_swizzle(int64_t lin, int64_t n)
 {
     const int64_t round = kXCDs * kXCDRun;
@@ -734,23 +786,7 @@ __device__ inline void gather_body(const GatherArgs &a, const int4 item, const i
     for (int cc = 0; cc < ncg; ++cc) {
         const float2 *g = a.grid + (int64_t)(cg0 + cc) * a.G;
         __syncthreads();
-        for (int rr = threadIdx.x; rr < RC; rr += kGatherThreads) {
-            const int r0 = rr / R12;
-            const int rem = rr - r0 * R12;
-            const int r1 = D == 3 ? rem / R2 : rem;
-            const int r2 = D == 3 ? rem - r1 * R2 : 0;
-            int g0 = org[0] + r0;
-            if (g0 >= a.nf[0]) g0 -= (int)a.nf[0];
-            int g1 = org[1] + r1;
-            if (g1 >= a.nf[1]) g1 -= (int)a.nf[1];
-            int64_t gi = (int64_t)g0 * a.nf[1] + g1;
-            if (D == 3) {
-                int g2 = org[2] + r2;
-                if (g2 >= a.nf[2]) g2 -= (int)a.nf[2];
-                gi = gi * a.nf[2] + g2;
-            }
-            lds_gather[rr] = g[gi];
-        }
+        stage_region<D>(lds_gather, g, a.nf, org, R1, R2, RC);
         __syncthreads();
         for (int j = item.y + threadIdx.x; j < item.z; j += kGatherThreads) {
             const int4 rc = a.rec_cell[j];
@@ -884,23 +920,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
         for (int64_t coil = 0; coil < a.nc; ++coil) {
             const float2 *g = a.grid + coil * a.G;
             __syncthreads();
-            for (int rr = threadIdx.x; rr < RC; rr += kGatherThreads) {
-                const int r0 = rr / R12;
-                const int rem = rr - r0 * R12;
-                const int r1 = D == 3 ? rem / R2 : rem;
-                const int r2 = D == 3 ? rem - r1 * R2 : 0;
-                int q0 = org[0] + r0;
-                if (q0 >= a.nf[0]) q0 -= (int)a.nf[0];
-                int q1 = org[1] + r1;
-                if (q1 >= a.nf[1]) q1 -= (int)a.nf[1];
-                int64_t gi = (int64_t)q0 * a.nf[1] + q1;
-                if (D == 3) {
-                    int q2 = org[2] + r2;
-                    if (q2 >= a.nf[2]) q2 -= (int)a.nf[2];
-                    gi = gi * a.nf[2] + q2;
-                }
-                lds_gather[rr] = g[gi];
-            }
+            stage_region<D>(lds_gather, g, a.nf, org, R1, R2, RC);
             __syncthreads();
             if (!valid) continue;
             // value sums with plain and derivative weights, separably

@@ -425,10 +425,14 @@ size_t row_lds(int L, int CL)
     return (size_t)L * sizeof(float2) + (size_t)CL * (L + 1) * sizeof(float2) + (size_t)L * sizeof(int);
 }
 
+#ifndef GN_FFT_COL_LDS_MAX
+#define GN_FFT_COL_LDS_MAX (64 * 1024)
+#endif
+
 int pick_cb(int L)
 {
     int cb = GN_FFT_CB;
-    while (cb > 1 && col_lds(L, cb) > 64 * 1024) cb /= 2;
+    while (cb > 1 && col_lds(L, cb) > GN_FFT_COL_LDS_MAX) cb /= 2;
     return cb;
 }
 
@@ -460,7 +464,10 @@ hipError_t col_pass(const FFTAxis &ax, float sign, float2 *data, int rows_in, in
     a.tw = ax.tw;
     a.pinv = ax.pinv;
     dim3 grid((unsigned)((ncols + a.CB - 1) / a.CB), (unsigned)nslabs);
-    hipLaunchKernelGGL(k_fft_cols, grid, dim3(kFFTColThreads), col_lds(ax.L, a.CB), st, a);
+    const size_t lds = col_lds(ax.L, a.CB);
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)k_fft_cols, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_fft_cols, grid, dim3(kFFTColThreads), lds, st, a);
     return hipGetLastError();
 }
 

@@ -44,6 +44,7 @@ struct FFTGroup {
 // instances that share its launch shape
 struct LaunchList {
     int ncg = 0, waves = 0, nbg = 0; // spread
+    bool mfma = false;               // spread on the matrix cores
     bool wrap = false;               // gather
     size_t lds = 0;
     std::vector<int4> items;
@@ -131,7 +132,7 @@ int upload_list(std::vector<int4> &h, int4 **d)
 int build_lists(gn_batch_s *b)
 {
     free_lists(b);
-    std::map<std::tuple<int, int, int, size_t>, size_t> skey;
+    std::map<std::tuple<int, bool, int, int, size_t>, size_t> skey;
     std::map<std::tuple<int, size_t>, size_t> gkey;
     for (int i = 0; i < b->n; ++i) {
         gn_plan_s *p = b->m[i];
@@ -140,12 +141,14 @@ int build_lists(gn_batch_s *b)
         const int ncg = p->nc >= 8 ? 8 : p->nc >= 4 ? 4 : p->nc >= 2 ? 2 : 1;
         int waves = 0, nbg = 0;
         spread_shape(sa, g.dim, &waves, &nbg);
-        const size_t slds = spread_lds(sa, ncg);
-        auto sk = std::make_tuple(ncg, waves, nbg, slds);
+        const bool mfma = spread_uses_mfma(sa, g.dim);
+        const size_t slds = spread_lds(sa, ncg, mfma, waves);
+        auto sk = std::make_tuple(ncg, mfma, waves, nbg, slds);
         if (!skey.count(sk)) {
             skey[sk] = b->spread_lists.size();
             LaunchList l;
             l.ncg = ncg;
+            l.mfma = mfma;
             l.waves = waves;
             l.nbg = nbg;
             l.lds = slds;
@@ -342,7 +345,7 @@ int enqueue(gn_batch_s *b, int kind, hipStream_t st)
     };
     auto type1 = [&]() -> int {
         for (auto &l : b->spread_lists)
-            if ((e = launch_spread_batch(b->dim, l.ncg, t.spread, l.d, (int)l.items.size(), l.waves, l.nbg, l.lds,
+            if ((e = launch_spread_batch(b->dim, l.ncg, l.mfma, t.spread, l.d, (int)l.items.size(), l.waves, l.nbg, l.lds,
                                          st)) != hipSuccess)
                 return hip_fail(e, "batch spread");
         if ((e = launch_spread_reduce_batch(b->dim, t.spread, b->d_reduce, (int)b->reduce_items.size(), st)) !=

@@ -221,9 +221,10 @@ SpreadArgs make_spread_args(const gn_plan_s *p, const float2 *samples, int use_d
 GatherArgs make_gather_args(const gn_plan_s *p, float2 *samples);
 PrepostArgs make_prepost(const gn_plan_s *p, float scale);
 void spread_shape(const SpreadArgs &a, int dim, int *waves, int *nbg);
-size_t spread_lds(const SpreadArgs &a, int ncg);
-hipError_t launch_spread_batch(int dim, int ncg, const SpreadArgs *d_args, const int4 *d_items, int n, int waves,
-                               int nbg, size_t lds, hipStream_t st);
+bool spread_uses_mfma(const SpreadArgs &a, int dim);
+size_t spread_lds(const SpreadArgs &a, int ncg, bool mfma, int waves);
+hipError_t launch_spread_batch(int dim, int ncg, bool mfma, const SpreadArgs *d_args, const int4 *d_items, int n,
+                               int waves, int nbg, size_t lds, hipStream_t st);
 hipError_t launch_spread_reduce_batch(int dim, const SpreadArgs *d_args, const int4 *d_items, int n, hipStream_t st);
 hipError_t launch_gather_batch(int dim, int ns, bool wrap, const GatherArgs *d_args, const int4 *d_items, int n,
                                size_t lds, hipStream_t st);

@@ -692,6 +692,306 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const Spr
     spread_body<D, NCG>(a, a.items[bi.y], bi.z, blockIdx.y);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint spread on the matrix cores (v_mfma_f32_16x16x4_f32).
+//
+// Same work items, pair staging and weight tables as spread_body; the per-wave
+// visit loop becomes small dense products.  A wave's 64-cell block is split into
+// four sub-blocks of 16 cells (2x2x4 in 3-D, 4x4 in 2-D).  Per batch, the wave
+// lists, for each sub-block, the staged pairs whose windows reach it (padded to
+// a multiple of 4 with an all-zero pair) and accumulates
+//     D[cell i][real j] += sum over 4 listed pairs k of  w_k(cell i) * v_k[j]
+// with one MFMA per 4 pairs: rows are the sub-block's cells, columns the 2*NCG
+// reals (re, im per coil; zero columns beyond), K the pairs.  A sub-block covers
+// on average ~half its cells per visit (a 64-cell block ~30%), and the FMAs run
+// on the matrix pipe.  The products are summed in a fixed order, so the result
+// is deterministic; batch sums are folded into Kahan-compensated registers as in
+// spread_body.  Lane layout of the 16x16x4 f32 operation: A[i][k] and B[k][j]
+// in lane i + 16k and j + 16k, D[4g + v][j] in register v of lane j + 16g.
+// Needs block-aligned tiles (multiples of 4 per axis in 3-D, of 8 in 2-D).
+// ---------------------------------------------------------------------------
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+#ifndef GN_SPREAD_UNROLL2
+#define GN_SPREAD_UNROLL2 0
+#endif
+
+inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
+{
+    return (size_t)(kSpreadBatch + 1) * a.TT * 4 + (size_t)(kSpreadBatch + 1) * 16 * 4 + (size_t)kSpreadBatch * 16 +
+           (size_t)kSpreadBatch * 32 + (size_t)waves * 4 * kSpreadBatch * 4;
+}
+
+template <int D, int NCG>
+__device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, const int cgi, const int bz)
+{
+    constexpr int B = kSpreadBatch;
+    extern __shared__ float lds_sp[];
+    // LDS: weight tables [B + 1][TT] | reals [B + 1][16] | masks [B][4] | records [B] int4, float4 |
+    //      per-wave pair lists [waves][4][B]; row B of the tables and reals is the all-zero pair
+    const int TT = a.TT;
+    float *s_tab = lds_sp;
+    float *s_val = s_tab + (B + 1) * TT;
+    int *s_msk = (int *)(s_val + (B + 1) * 16);
+    int4 *s_pa = (int4 *)(s_msk + B * 4);
+    float4 *s_pb = (float4 *)(s_pa + B);
+    int *s_list = (int *)(s_pb + B);
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int nwaves = blockDim.x >> 6;
+    const int cg0 = cgi * NCG;
+    const int ncg = (int)min((int64_t)NCG, a.nc - cg0);
+    const int T0 = a.tile[0], T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
+    const int to1 = T0, to2 = T0 + T1; // table offsets of axes 1 and 2
+
+    const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
+    const int bidx = bz * nwaves + wave;
+    const bool has_block = bidx < nb;
+    int bq0, bq1, bq2 = 0;
+    if (D == 3) {
+        bq2 = bidx % a.nblk[2];
+        bq1 = (bidx / a.nblk[2]) % a.nblk[1];
+        bq0 = bidx / (a.nblk[2] * a.nblk[1]);
+    } else {
+        bq1 = bidx % a.nblk[1];
+        bq0 = bidx / a.nblk[1];
+    }
+    // this lane's row (cell) within a sub-block and its pair slot k = lane / 16
+    const int ri = lane & 15, kg = lane >> 4;
+    // tile-local origin of sub-block 0 and the row's offset inside a sub-block;
+    // sub-block s adds (SB0 * (s >> 1), SB1 * (s & 1)) on axes 0 and 1
+    constexpr int SB0 = D == 3 ? 2 : 4, SB1 = D == 3 ? 2 : 4;
+    const int X0 = (D == 3 ? bq0 * 4 : bq0 * 8) + (D == 3 ? (ri >> 3) : (ri >> 2));
+    const int Y0 = (D == 3 ? bq1 * 4 : bq1 * 8) + (D == 3 ? ((ri >> 2) & 1) : (ri & 3));
+    const int Zc = D == 3 ? bq2 * 4 + (ri & 3) : 0;
+    const uint32_t sbm2 = D == 3 ? (0xfu << (bq2 * 4)) : 1u;
+
+    if (tid < TT) s_tab[B * TT + tid] = 0.f;
+    if (tid < 16) s_val[B * 16 + tid] = 0.f;
+
+    float S[4][4], K[4][4];
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) S[sb][v] = K[sb][v] = 0.f;
+
+    const float rTT = 1.0f / (float)TT;
+    for (int base = item.y; base < item.z; base += B) {
+        const int n = min(B, item.z - base);
+        __syncthreads(); // previous batch fully consumed
+        // phase 1: one thread per pair loads its record and coil values, builds the masks
+        if (tid < n) {
+            const int q = base + tid;
+            const int4 pa = a.pair_a[q];
+            const int4 pbi = a.pair_b[q];
+            const float4 pb = make_float4(__int_as_float(pbi.x), __int_as_float(pbi.y), __int_as_float(pbi.z),
+                                          __int_as_float(pbi.w));
+            s_pa[tid] = pa;
+            s_pb[tid] = pb;
+            const float scale = a.use_dcf ? pb.w : 1.0f;
+            float2 v[NCG];
+#pragma unroll
+            for (int c = 0; c < NCG; ++c) {
+                v[c] = make_float2(0.f, 0.f);
+                if (c < ncg) {
+                    const int64_t coil = cg0 + c;
+                    v[c] = (a.layout == GN_COIL_LAST) ? a.in[(int64_t)pa.x * a.nc + coil] : a.in[coil * a.M + pa.x];
+                }
+            }
+            float4 *dv = (float4 *)(s_val + tid * 16);
+#pragma unroll
+            for (int c2 = 0; c2 < 4; ++c2) {
+                float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (2 * c2 < NCG) o = make_float4(v[2 * c2].x * scale, v[2 * c2].y * scale, 0.f, 0.f);
+                if (2 * c2 + 1 < NCG) {
+                    o.z = v[2 * c2 + 1].x * scale;
+                    o.w = v[2 * c2 + 1].y * scale;
+                }
+                dv[c2] = o;
+            }
+            // valid tile-local cells of the window per axis: (L - l) mod T < c
+            const int Ts[3] = {T0, T1, T2};
+            for (int ax = 0; ax < D; ++ax) {
+                const int l = (ax == 0 ? pa.w : ax == 1 ? (pa.w >> 10) : (pa.w >> 20)) & 0x3ff;
+                const int c = (pa.z >> (8 * ax)) & 0xff;
+                uint32_t m = 0;
+                for (int r = 0; r < c; ++r) {
+                    int L = l + r;
+                    if (L >= Ts[ax]) L -= Ts[ax];
+                    m |= 1u << L;
+                }
+                s_msk[tid * 4 + ax] = (int)m;
+            }
+        }
+        __syncthreads();
+        // phase 2: weight tables, one entry per (pair, axis, tile-local cell)
+        for (int e = tid; e < n * TT; e += blockDim.x) {
+            const int k = fdiv_small(e, TT, rTT);
+            const int i = e - k * TT;
+            const int ax = i < to1 ? 0 : (i < to2 ? 1 : 2);
+            const int L = i - (ax == 0 ? 0 : (ax == 1 ? to1 : to2));
+            const int4 pa = s_pa[k];
+            const float4 pb = s_pb[k];
+            const int Tax = ax == 0 ? T0 : (ax == 1 ? T1 : T2);
+            const int l = (ax == 0 ? pa.w : ax == 1 ? (pa.w >> 10) : (pa.w >> 20)) & 0x3ff;
+            const int c = (pa.z >> (8 * ax)) & 0xff;
+            const int j = (pa.y >> (8 * ax)) & 0xff;
+            int rel = L - l;
+            if (rel < 0) rel += Tax;
+            float w = 0.f;
+            if (rel < c) {
+                const float t = ax == 0 ? pb.x : (ax == 1 ? pb.y : pb.z);
+                const float u = t + (float)(a.ns - j - rel);
+                w = exp2f(-a.coef[ax] * u * u);
+            }
+            s_tab[e] = w;
+        }
+        __syncthreads();
+        if (!has_block) continue;
+        // phase 3: per sub-block, list the pairs that reach it, then 4 pairs per MFMA
+        uint32_t m0 = 0, m1 = 0, m2 = 0;
+        if (lane < n) {
+            m0 = (uint32_t)s_msk[lane * 4];
+            m1 = (uint32_t)s_msk[lane * 4 + 1];
+            m2 = D == 3 ? (uint32_t)s_msk[lane * 4 + 2] : 1u;
+        }
+        int cnt4[4];
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const uint32_t bm0 = (D == 3 ? 0x3u : 0xfu) << ((D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1));
+            const uint32_t bm1 = (D == 3 ? 0x3u : 0xfu) << ((D == 3 ? bq1 * 4 : bq1 * 8) + SB1 * (sb & 1));
+            const bool hit = (m0 & bm0) && (m1 & bm1) && (m2 & sbm2);
+            const unsigned long long hm = __ballot(hit);
+            const int cnt = __popcll(hm);
+            int *lst = s_list + (wave * 4 + sb) * B;
+            const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0));
+            if (hit) lst[pos] = lane;
+            cnt4[sb] = (cnt + 3) & ~3;
+            if (lane >= cnt && lane < cnt4[sb]) lst[lane] = B;
+        }
+        // the lists were written by this wave's own lanes: order the LDS accesses
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const int *lst = s_list + (wave * 4 + sb) * B;
+            const int X = X0 + SB0 * (sb >> 1), Y = Y0 + SB1 * (sb & 1);
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            int t = 0;
+#if GN_SPREAD_UNROLL2
+            // two independent steps per iteration: their LDS reads overlap
+            f32x4 acc1 = {0.f, 0.f, 0.f, 0.f};
+            for (; t + 8 <= cnt4[sb]; t += 8) {
+                const int ka = lst[t + kg], kb = lst[t + 4 + kg];
+                const float *ta = s_tab + ka * TT, *tb = s_tab + kb * TT;
+                float wa = ta[X] * ta[to1 + Y], wb = tb[X] * tb[to1 + Y];
+                if (D == 3) {
+                    wa *= ta[to2 + Zc];
+                    wb *= tb[to2 + Zc];
+                }
+                const float va = s_val[ka * 16 + ri], vb = s_val[kb * 16 + ri];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wa, va, acc, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(wb, vb, acc1, 0, 0, 0);
+            }
+#endif
+            for (; t < cnt4[sb]; t += 4) {
+                const int kk = lst[t + kg];
+                const float *tab = s_tab + kk * TT;
+                float w = tab[X] * tab[to1 + Y];
+                if (D == 3) w *= tab[to2 + Zc];
+                const float v = s_val[kk * 16 + ri];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(w, v, acc, 0, 0, 0);
+            }
+#if GN_SPREAD_UNROLL2
+            acc += acc1;
+#endif
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float y = acc[v] - K[sb][v];
+                const float tsum = S[sb][v] + y;
+                K[sb][v] = (tsum - S[sb][v]) - y;
+                S[sb][v] = tsum;
+            }
+        }
+    }
+    if (!has_block) return;
+
+    // write: lane (j + 16g) holds real j of cells 4g..4g+3 of each sub-block, which
+    // are consecutive along the last axis; lanes j and j^1 swap halves so that each
+    // writes two whole complex cells
+    const int j = ri;
+    const int coil = j >> 1;
+    const bool im = j & 1;
+    int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
+    if (item.w < 0) {
+        int64_t r = item.x;
+        for (int ax = D - 1; ax >= 0; --ax) {
+            const int qq = (int)(r % a.ntiles[ax]);
+            r /= a.ntiles[ax];
+            org[ax] = qq * a.tile[ax];
+            teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
+        }
+    }
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+        float mine[4], other[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            mine[v] = S[sb][v] - K[sb][v];
+            other[v] = __shfl_xor(mine[v], 1);
+        }
+        if (coil >= ncg) continue;
+        // even lanes write cells 4g, 4g+1; odd lanes 4g+2, 4g+3
+        const int v0 = im ? 2 : 0;
+        float2 val[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = v0 + h;
+            const float re = im ? other[v] : mine[v];
+            const float iv = im ? mine[v] : other[v];
+            val[h] = make_float2(re, iv);
+        }
+        const int X = X0 - (D == 3 ? (ri >> 3) : (ri >> 2)) + SB0 * (sb >> 1);
+        const int Y = Y0 - (D == 3 ? ((ri >> 2) & 1) : (ri & 3)) + SB1 * (sb & 1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int cell = 4 * kg + v0 + h; // row within the sub-block
+            const int L0 = X + (D == 3 ? (cell >> 3) : (cell >> 2));
+            const int L1 = Y + (D == 3 ? ((cell >> 2) & 1) : (cell & 3));
+            const int L2 = D == 3 ? bq2 * 4 + (cell & 3) : 0;
+            const int cidx = (L0 * T1 + L1) * T2 + L2;
+            if (item.w >= 0) {
+                a.partial[((int64_t)item.w * a.nc + cg0 + coil) * a.TC + cidx] = val[h];
+            } else if (L0 < teff[0] && L1 < teff[1] && (D == 2 || L2 < teff[2])) {
+                int64_t gi = (int64_t)(org[0] + L0) * a.nf[1] + (org[1] + L1);
+                if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
+                a.grid[(int64_t)(cg0 + coil) * a.G + gi] = val[h];
+            }
+        }
+    }
+}
+
+template <int D, int NCG>
+__global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma(SpreadArgs a)
+{
+    const int64_t n = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t lin = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+    const int64_t w = xcd_swizzle(lin, n);
+    const int bx = (int)(w % gridDim.x);
+    const int64_t yz = w / gridDim.x;
+    spread_mfma_body<D, NCG>(a, a.items[bx], (int)(yz % gridDim.y), (int)(yz / gridDim.y));
+}
+
+template <int D, int NCG>
+__global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma_batch(const SpreadArgs *__restrict__ args,
+                                                                             const int4 *__restrict__ bitems)
+{
+    const int4 bi = bitems[blockIdx.x];
+    const SpreadArgs a = args[bi.x];
+    spread_mfma_body<D, NCG>(a, a.items[bi.y], bi.z, blockIdx.y);
+}
+
 // Kahan-compensated sum of the partial tiles of each split tile, in slice order.
 template <int D>
 __device__ inline void spread_reduce_body(const SpreadArgs &a, const int4 it)
@@ -1380,6 +1680,20 @@ void lds_limit(K kernel, size_t lds)
     if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+#ifndef GN_SPREAD_MFMA
+#define GN_SPREAD_MFMA 1
+#endif
+// The matrix-core spread covers whole 64-cell blocks: tiles must be multiples of the
+// block (4 per axis in 3-D, 8 in 2-D).
+bool mfma_tiles_ok(const SpreadArgs &a, int dim)
+{
+    if (!GN_SPREAD_MFMA) return false;
+    const int q = dim == 3 ? 4 : 8;
+    for (int ax = 0; ax < dim; ++ax)
+        if (a.blk[ax] != q || a.tile[ax] % q != 0) return false;
+    return true;
+}
+
 inline size_t spread_lds_bytes(const SpreadArgs &a, int ncg)
 {
     return (size_t)kSpreadBatch * a.TT * 4 + (size_t)kSpreadBatch * ncg * 8 + (size_t)kSpreadBatch * 16 +
@@ -1393,10 +1707,16 @@ hipError_t spread_launch(const SpreadArgs &a, int nitems, int64_t nc, hipStream_
     const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
     const int waves = std::min(nb, kSpreadWavesMax);
     const int nbg = (nb + waves - 1) / waves;
-    const size_t lds = spread_lds_bytes(a, NCG);
-    lds_limit(k_spread<D, NCG>, lds);
-    hipLaunchKernelGGL((k_spread<D, NCG>), dim3((unsigned)nitems, (unsigned)ng, (unsigned)nbg), dim3(64 * waves), lds,
-                       st, a);
+    const dim3 grid((unsigned)nitems, (unsigned)ng, (unsigned)nbg);
+    if (mfma_tiles_ok(a, D)) {
+        const size_t lds = spread_mfma_lds_bytes(a, waves);
+        lds_limit(k_spread_mfma<D, NCG>, lds);
+        hipLaunchKernelGGL((k_spread_mfma<D, NCG>), grid, dim3(64 * waves), lds, st, a);
+    } else {
+        const size_t lds = spread_lds_bytes(a, NCG);
+        lds_limit(k_spread<D, NCG>, lds);
+        hipLaunchKernelGGL((k_spread<D, NCG>), grid, dim3(64 * waves), lds, st, a);
+    }
     return hipGetLastError();
 }
 
@@ -1681,16 +2001,27 @@ void spread_shape(const SpreadArgs &a, int dim, int *waves, int *nbg)
     *nbg = (nb + *waves - 1) / *waves;
 }
 
-size_t spread_lds(const SpreadArgs &a, int ncg) { return spread_lds_bytes(a, ncg); }
+bool spread_uses_mfma(const SpreadArgs &a, int dim) { return mfma_tiles_ok(a, dim); }
+
+size_t spread_lds(const SpreadArgs &a, int ncg, bool mfma, int waves)
+{
+    return mfma ? spread_mfma_lds_bytes(a, waves) : spread_lds_bytes(a, ncg);
+}
 
 namespace {
 template <int D, int NCG>
-hipError_t spread_batch_launch(const SpreadArgs *d_args, const int4 *d_items, int n, int waves, int nbg, size_t lds,
-                               hipStream_t st)
+hipError_t spread_batch_launch(bool mfma, const SpreadArgs *d_args, const int4 *d_items, int n, int waves, int nbg,
+                               size_t lds, hipStream_t st)
 {
-    lds_limit(k_spread_batch<D, NCG>, lds);
-    hipLaunchKernelGGL((k_spread_batch<D, NCG>), dim3((unsigned)n, (unsigned)nbg), dim3(64 * waves), lds, st, d_args,
-                       d_items);
+    if (mfma) {
+        lds_limit(k_spread_mfma_batch<D, NCG>, lds);
+        hipLaunchKernelGGL((k_spread_mfma_batch<D, NCG>), dim3((unsigned)n, (unsigned)nbg), dim3(64 * waves), lds, st,
+                           d_args, d_items);
+    } else {
+        lds_limit(k_spread_batch<D, NCG>, lds);
+        hipLaunchKernelGGL((k_spread_batch<D, NCG>), dim3((unsigned)n, (unsigned)nbg), dim3(64 * waves), lds, st,
+                           d_args, d_items);
+    }
     return hipGetLastError();
 }
 
@@ -1711,23 +2042,23 @@ hipError_t gather_batch_dispatch(bool wrap, const GatherArgs *d_args, const int4
 }
 } // namespace
 
-hipError_t launch_spread_batch(int dim, int ncg, const SpreadArgs *d_args, const int4 *d_items, int n, int waves,
-                               int nbg, size_t lds, hipStream_t st)
+hipError_t launch_spread_batch(int dim, int ncg, bool mfma, const SpreadArgs *d_args, const int4 *d_items, int n,
+                               int waves, int nbg, size_t lds, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
     if (dim == 3) {
         switch (ncg) {
-        case 8: return spread_batch_launch<3, 8>(d_args, d_items, n, waves, nbg, lds, st);
-        case 4: return spread_batch_launch<3, 4>(d_args, d_items, n, waves, nbg, lds, st);
-        case 2: return spread_batch_launch<3, 2>(d_args, d_items, n, waves, nbg, lds, st);
-        case 1: return spread_batch_launch<3, 1>(d_args, d_items, n, waves, nbg, lds, st);
+        case 8: return spread_batch_launch<3, 8>(mfma, d_args, d_items, n, waves, nbg, lds, st);
+        case 4: return spread_batch_launch<3, 4>(mfma, d_args, d_items, n, waves, nbg, lds, st);
+        case 2: return spread_batch_launch<3, 2>(mfma, d_args, d_items, n, waves, nbg, lds, st);
+        case 1: return spread_batch_launch<3, 1>(mfma, d_args, d_items, n, waves, nbg, lds, st);
         }
     } else {
         switch (ncg) {
-        case 8: return spread_batch_launch<2, 8>(d_args, d_items, n, waves, nbg, lds, st);
-        case 4: return spread_batch_launch<2, 4>(d_args, d_items, n, waves, nbg, lds, st);
-        case 2: return spread_batch_launch<2, 2>(d_args, d_items, n, waves, nbg, lds, st);
-        case 1: return spread_batch_launch<2, 1>(d_args, d_items, n, waves, nbg, lds, st);
+        case 8: return spread_batch_launch<2, 8>(mfma, d_args, d_items, n, waves, nbg, lds, st);
+        case 4: return spread_batch_launch<2, 4>(mfma, d_args, d_items, n, waves, nbg, lds, st);
+        case 2: return spread_batch_launch<2, 2>(mfma, d_args, d_items, n, waves, nbg, lds, st);
+        case 1: return spread_batch_launch<2, 1>(mfma, d_args, d_items, n, waves, nbg, lds, st);
         }
     }
     return hipErrorInvalidValue;

@@ -719,7 +719,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
 {
     return (size_t)(kSpreadBatch + 1) * a.TT * 4 + (size_t)(kSpreadBatch + 1) * 16 * 4 + (size_t)kSpreadBatch * 16 +
-           (size_t)kSpreadBatch * 32 + (size_t)waves * 4 * kSpreadBatch * 4;
+           (size_t)kSpreadBatch * 64 + (size_t)waves * 4 * kSpreadBatch * 4;
 }
 
 template <int D, int NCG>
@@ -727,15 +727,15 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
 {
     constexpr int B = kSpreadBatch;
     extern __shared__ float lds_sp[];
-    // LDS: weight tables [B + 1][TT] | reals [B + 1][16] | masks [B][4] | records [B] int4, float4 |
-    //      per-wave pair lists [waves][4][B]; row B of the tables and reals is the all-zero pair
+    // LDS: weight tables [B + 1][TT] | reals [B + 1][16] | masks [B][4] | per-axis window
+    //      parameters [B][4] {l, c, ns - j, t} | per-wave pair lists [waves][4][B];
+    //      row B of the tables and reals is the all-zero pair
     const int TT = a.TT;
     float *s_tab = lds_sp;
     float *s_val = s_tab + (B + 1) * TT;
     int *s_msk = (int *)(s_val + (B + 1) * 16);
-    int4 *s_pa = (int4 *)(s_msk + B * 4);
-    float4 *s_pb = (float4 *)(s_pa + B);
-    int *s_list = (int *)(s_pb + B);
+    int4 *s_par = (int4 *)(s_msk + B * 4);
+    int *s_list = (int *)(s_par + B * 4);
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -777,7 +777,17 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
 #pragma unroll
         for (int v = 0; v < 4; ++v) S[sb][v] = K[sb][v] = 0.f;
 
-    const float rTT = 1.0f / (float)TT;
+    // phase-2 entry of this thread: table column e_i (axis e_ax, cell e_L) of pairs
+    // e_k0, e_k0 + e_kstep, ...; columns are padded to a power of two (the workgroup
+    // has at least 64 threads and TT <= 96, so at least one pair per step)
+    const int ttp = TT <= 16 ? 16 : TT <= 32 ? 32 : TT <= 64 ? 64 : 128;
+    const int e_i = tid & (ttp - 1);
+    const int e_k0 = tid / ttp;
+    const int e_kstep = max(1, (int)blockDim.x / ttp);
+    const int e_ax = e_i < to1 ? 0 : (e_i < to2 ? 1 : 2);
+    const int e_L = e_i - (e_ax == 0 ? 0 : (e_ax == 1 ? to1 : to2));
+    const int e_T = e_ax == 0 ? T0 : (e_ax == 1 ? T1 : T2);
+    const float e_coef = a.coef[e_ax];
     for (int base = item.y; base < item.z; base += B) {
         const int n = min(B, item.z - base);
         __syncthreads(); // previous batch fully consumed
@@ -788,8 +798,6 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
             const int4 pbi = a.pair_b[q];
             const float4 pb = make_float4(__int_as_float(pbi.x), __int_as_float(pbi.y), __int_as_float(pbi.z),
                                           __int_as_float(pbi.w));
-            s_pa[tid] = pa;
-            s_pb[tid] = pb;
             const float scale = a.use_dcf ? pb.w : 1.0f;
             float2 v[NCG];
 #pragma unroll
@@ -811,42 +819,36 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
                 }
                 dv[c2] = o;
             }
-            // valid tile-local cells of the window per axis: (L - l) mod T < c
+            // per axis: the window's tile-local cells (L - l) mod T < c as a bit mask,
+            // and the parameters of its weights
             const int Ts[3] = {T0, T1, T2};
+            const float ts[3] = {pb.x, pb.y, pb.z};
+#pragma unroll
             for (int ax = 0; ax < D; ++ax) {
                 const int l = (ax == 0 ? pa.w : ax == 1 ? (pa.w >> 10) : (pa.w >> 20)) & 0x3ff;
                 const int c = (pa.z >> (8 * ax)) & 0xff;
-                uint32_t m = 0;
-                for (int r = 0; r < c; ++r) {
-                    int L = l + r;
-                    if (L >= Ts[ax]) L -= Ts[ax];
-                    m |= 1u << L;
-                }
-                s_msk[tid * 4 + ax] = (int)m;
+                const int j = (pa.y >> (8 * ax)) & 0xff;
+                uint64_t m = ((1ull << c) - 1) << l;
+                m = (m | (m >> Ts[ax])) & ((1ull << Ts[ax]) - 1);
+                s_msk[tid * 4 + ax] = (int)(uint32_t)m;
+                s_par[tid * 4 + ax] = make_int4(l, c, a.ns - j, __float_as_int(ts[ax]));
             }
         }
         __syncthreads();
-        // phase 2: weight tables, one entry per (pair, axis, tile-local cell)
-        for (int e = tid; e < n * TT; e += blockDim.x) {
-            const int k = fdiv_small(e, TT, rTT);
-            const int i = e - k * TT;
-            const int ax = i < to1 ? 0 : (i < to2 ? 1 : 2);
-            const int L = i - (ax == 0 ? 0 : (ax == 1 ? to1 : to2));
-            const int4 pa = s_pa[k];
-            const float4 pb = s_pb[k];
-            const int Tax = ax == 0 ? T0 : (ax == 1 ? T1 : T2);
-            const int l = (ax == 0 ? pa.w : ax == 1 ? (pa.w >> 10) : (pa.w >> 20)) & 0x3ff;
-            const int c = (pa.z >> (8 * ax)) & 0xff;
-            const int j = (pa.y >> (8 * ax)) & 0xff;
-            int rel = L - l;
-            if (rel < 0) rel += Tax;
-            float w = 0.f;
-            if (rel < c) {
-                const float t = ax == 0 ? pb.x : (ax == 1 ? pb.y : pb.z);
-                const float u = t + (float)(a.ns - j - rel);
-                w = exp2f(-a.coef[ax] * u * u);
+        // phase 2: weight tables, one entry per (pair, axis, tile-local cell); a thread
+        // keeps one entry index i (its axis and cell) and steps over the pairs
+        if (e_i < TT) {
+            for (int k = e_k0; k < n; k += e_kstep) {
+                const int4 par = s_par[k * 4 + e_ax];
+                int rel = e_L - par.x;
+                if (rel < 0) rel += e_T;
+                float w = 0.f;
+                if (rel < par.y) {
+                    const float u = __int_as_float(par.w) + (float)(par.z - rel);
+                    w = __builtin_amdgcn_exp2f(-e_coef * u * u);
+                }
+                s_tab[k * TT + e_i] = w;
             }
-            s_tab[e] = w;
         }
         __syncthreads();
         if (!has_block) continue;
@@ -919,20 +921,24 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
 
     // write: lane (j + 16g) holds real j of cells 4g..4g+3 of each sub-block, which
     // are consecutive along the last axis; lanes j and j^1 swap halves so that each
-    // writes two whole complex cells
+    // holds two whole complex cells (cells 4g, 4g+1 on even lanes, 4g+2, 4g+3 on odd)
     const int j = ri;
     const int coil = j >> 1;
     const bool im = j & 1;
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     if (item.w < 0) {
-        int64_t r = item.x;
+        int r = item.x;
         for (int ax = D - 1; ax >= 0; --ax) {
-            const int qq = (int)(r % a.ntiles[ax]);
-            r /= a.ntiles[ax];
+            const int nt = a.ntiles[ax];
+            const int qq = r % nt;
+            r /= nt;
             org[ax] = qq * a.tile[ax];
             teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
     }
+    const int nflast = (int)a.nf[D - 1];
+    // 16-byte stores need an even cell offset: even grid size and last-axis length
+    const bool wide = ((a.G | nflast) & 1) == 0;
 #pragma unroll
     for (int sb = 0; sb < 4; ++sb) {
         float mine[4], other[4];
@@ -942,32 +948,35 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
             other[v] = __shfl_xor(mine[v], 1);
         }
         if (coil >= ncg) continue;
-        // even lanes write cells 4g, 4g+1; odd lanes 4g+2, 4g+3
         const int v0 = im ? 2 : 0;
-        float2 val[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int v = v0 + h;
-            const float re = im ? other[v] : mine[v];
-            const float iv = im ? mine[v] : other[v];
-            val[h] = make_float2(re, iv);
+        float4 val;
+        val.x = im ? other[v0] : mine[v0];
+        val.y = im ? mine[v0] : other[v0];
+        val.z = im ? other[v0 + 1] : mine[v0 + 1];
+        val.w = im ? mine[v0 + 1] : other[v0 + 1];
+        // first of the two cells: row 4g + v0 of the sub-block
+        const int cell = 4 * kg + v0;
+        const int L0 = (D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1) + (D == 3 ? (cell >> 3) : (cell >> 2));
+        const int L1 = (D == 3 ? bq1 * 4 : bq1 * 8) + SB1 * (sb & 1) + (D == 3 ? ((cell >> 2) & 1) : (cell & 3));
+        const int L2 = D == 3 ? bq2 * 4 + (cell & 3) : 0;
+        const int cidx = (L0 * T1 + L1) * T2 + L2;
+        if (item.w >= 0) {
+            float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0 + coil) * a.TC + cidx;
+            dst[0] = make_float2(val.x, val.y);
+            dst[1] = make_float2(val.z, val.w);
+            continue;
         }
-        const int X = X0 - (D == 3 ? (ri >> 3) : (ri >> 2)) + SB0 * (sb >> 1);
-        const int Y = Y0 - (D == 3 ? ((ri >> 2) & 1) : (ri & 3)) + SB1 * (sb & 1);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int cell = 4 * kg + v0 + h; // row within the sub-block
-            const int L0 = X + (D == 3 ? (cell >> 3) : (cell >> 2));
-            const int L1 = Y + (D == 3 ? ((cell >> 2) & 1) : (cell & 3));
-            const int L2 = D == 3 ? bq2 * 4 + (cell & 3) : 0;
-            const int cidx = (L0 * T1 + L1) * T2 + L2;
-            if (item.w >= 0) {
-                a.partial[((int64_t)item.w * a.nc + cg0 + coil) * a.TC + cidx] = val[h];
-            } else if (L0 < teff[0] && L1 < teff[1] && (D == 2 || L2 < teff[2])) {
-                int64_t gi = (int64_t)(org[0] + L0) * a.nf[1] + (org[1] + L1);
-                if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
-                a.grid[(int64_t)(cg0 + coil) * a.G + gi] = val[h];
-            }
+        if (L0 >= teff[0] || (D == 3 && L1 >= teff[1])) continue;
+        const int Llast = D == 3 ? L2 : L1;          // position along the last axis
+        const int tlast = teff[D - 1];
+        int64_t gi = (int64_t)(org[0] + L0) * a.nf[1] + (org[1] + L1);
+        if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
+        float2 *dst = a.grid + (int64_t)(cg0 + coil) * a.G + gi;
+        if (wide && Llast + 1 < tlast) {
+            *(float4 *)dst = val;
+        } else {
+            if (Llast < tlast) dst[0] = make_float2(val.x, val.y);
+            if (Llast + 1 < tlast) dst[1] = make_float2(val.z, val.w);
         }
     }
 }
@@ -1689,9 +1698,14 @@ bool mfma_tiles_ok(const SpreadArgs &a, int dim)
 {
     if (!GN_SPREAD_MFMA) return false;
     const int q = dim == 3 ? 4 : 8;
-    for (int ax = 0; ax < dim; ++ax)
+    int nb = 1;
+    for (int ax = 0; ax < dim; ++ax) {
         if (a.blk[ax] != q || a.tile[ax] % q != 0) return false;
-    return true;
+        nb *= a.tile[ax] / q;
+    }
+    // phase 2 maps one table column per thread: 64 * waves >= TT rounded up to a power of 2
+    const int ttp = a.TT <= 16 ? 16 : a.TT <= 32 ? 32 : a.TT <= 64 ? 64 : 128;
+    return a.TT <= 128 && 64 * std::min(nb, kSpreadWavesMax) >= ttp;
 }
 
 inline size_t spread_lds_bytes(const SpreadArgs &a, int ncg)

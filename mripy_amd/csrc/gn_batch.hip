@@ -434,7 +434,7 @@ std::vector<std::vector<int>> candidates(int dim)
 {
     std::vector<std::vector<int>> out;
     if (dim == 2) {
-        const int sp[][2] = {{16, 32}, {8, 32}, {16, 16}, {32, 32}};
+        const int sp[][2] = {{8, 16}, {16, 8}, {16, 16}, {8, 32}};
         const int ga[][2] = {{32, 32}, {16, 32}, {32, 64}, {64, 64}};
         for (auto &s : sp)
             for (auto &g : ga) out.push_back({s[0], s[1], 1, g[0], g[1], 1});

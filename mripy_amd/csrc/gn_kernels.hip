@@ -1514,8 +1514,8 @@ void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
         gtile[1] = 8;
         gtile[2] = 16;
     } else {
-        tile[0] = 16;
-        tile[1] = 32;
+        tile[0] = 8;
+        tile[1] = 16;
         tile[2] = 1;
         gtile[0] = 32;
         gtile[1] = 32;

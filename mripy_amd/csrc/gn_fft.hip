@@ -44,6 +44,7 @@ namespace {
 constexpr int kFFTThreads = GN_FFT_THREADS;       // row passes
 constexpr int kLoadUnroll = 8;                    // global loads in flight per thread
 constexpr int kFFTColThreads = GN_FFT_COL_THREADS; // column passes
+static_assert((GN_FFT_CB & (GN_FFT_CB - 1)) == 0, "column blocks index rows with shifts");
 
 struct Stages {
     int n;
@@ -57,7 +58,8 @@ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x
 __device__ inline float2 mul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
 
 // One radix-r butterfly on LDS elements at idx0 + k*step_q (k < r) of a line, after
-// twiddling element k by w_g^{t k}: tw[(t k L / g)] (conjugated for sign > 0).
+// twiddling element k by w_g^{t k}: tw[(t k L / g)] (the table in LDS already holds
+// the twiddles of this transform's sign).
 template <int R>
 __device__ inline void butterfly(float2 *buf, int base, int qstride, const float2 *tw, int twstep, int t, float sign)
 {
@@ -66,9 +68,7 @@ __device__ inline void butterfly(float2 *buf, int base, int qstride, const float
     for (int k = 0; k < R; ++k) {
         float2 a = buf[base + k * qstride];
         if (k > 0 && t > 0) {
-            float2 w = tw[t * k * twstep];
-            w.y *= -sign; // table holds e^{-2 pi i k/L}; sign -1 keeps it, +1 conjugates
-            a = cmul(a, w);
+            a = cmul(a, tw[t * k * twstep]);
         }
         v[k] = a;
     }
@@ -124,6 +124,8 @@ __device__ inline void fft_lines(float2 *buf, int nlines, int lstride, int estri
                                  const float2 *tw, float sign)
 {
     const float rl = 1.0f / (float)nlines;
+    // power-of-two line counts and butterfly groups index with shifts
+    const int lgl = (nlines & (nlines - 1)) == 0 ? __builtin_ctz(nlines) : -1;
     int q = 1;
     for (int s = 0; s < st.n; ++s) {
         const int r = st.radix[s];
@@ -131,11 +133,12 @@ __device__ inline void fft_lines(float2 *buf, int nlines, int lstride, int estri
         const int twstep = L / g;
         const int nb = L / r; // butterflies per line
         const float rq = 1.0f / (float)q;
+        const int lgq = (q & (q - 1)) == 0 ? __builtin_ctz(q) : -1;
         for (int w = threadIdx.x; w < nb * nlines; w += blockDim.x) {
             // lines fastest: neighbouring threads work on neighbouring lines
-            const int bidx = fdiv(w, nlines, rl);
+            const int bidx = lgl >= 0 ? (w >> lgl) : fdiv(w, nlines, rl);
             const int j = w - bidx * nlines;
-            const int blk = fdiv(bidx, q, rq);
+            const int blk = lgq >= 0 ? (bidx >> lgq) : fdiv(bidx, q, rq);
             const int t = bidx - blk * q;
             const int base = j * lstride + (blk * g + t) * estride;
             switch (r) {
@@ -150,9 +153,14 @@ __device__ inline void fft_lines(float2 *buf, int nlines, int lstride, int estri
     }
 }
 
-__device__ inline void load_table(float2 *dst, const float2 *src, int n)
+// Twiddles of a transform of this sign: the table holds e^{-2 pi i k/L}; sign -1
+// keeps it, +1 conjugates it.
+__device__ inline void load_table(float2 *dst, const float2 *src, int n, float sign)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const float2 w = src[i];
+        dst[i] = make_float2(w.x, -sign * w.y);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -190,13 +198,13 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
     const int ncb = (int)min((int64_t)CB, a.ncols - col0);
     float2 *base = a.data + (slab / a.slab_inner) * a.stride_outer + (slab % a.slab_inner) * a.stride_inner + col0;
     int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
-    load_table(tw, a.tw, a.L);
+    load_table(tw, a.tw, a.L, a.sign);
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     if (a.rows_in < a.L) // pruned input: zero every slot first
         for (int e = threadIdx.x; e < a.L * estride; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
     const int half_in = a.rows_in / 2;
-    const float rcb = 1.0f / (float)CB;
+    const int cbs = __builtin_ctz(CB); // CB is a power of two
     const int total_in = a.rows_in * CB;
     // kLoadUnroll independent global loads in flight per thread before the LDS stores
     for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
@@ -204,13 +212,13 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;
-            const int n = fdiv(e, CB, rcb), j = e - n * CB;
+            const int n = e >> cbs, j = e & (CB - 1);
             if (e < total_in && j < ncb) v[u] = base[(int64_t)n * a.ncols + j];
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;
-            const int n = fdiv(e, CB, rcb), j = e - n * CB;
+            const int n = e >> cbs, j = e & (CB - 1);
             if (e < total_in && j < ncb) {
                 int p = n;
                 if (a.in_map) {
@@ -225,7 +233,7 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
     fft_lines(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
     const int half_out = a.rows_out / 2;
     for (int e = threadIdx.x; e < a.rows_out * CB; e += blockDim.x) {
-        const int n = fdiv(e, CB, rcb), j = e - n * CB;
+        const int n = e >> cbs, j = e & (CB - 1);
         if (j >= ncb) continue;
         int p = n;
         if (a.out_map) {
@@ -275,7 +283,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a)
     const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
     const int ls = a.L + 1;
     int *spinv = (int *)(buf + a.CL * ls);
-    load_table(tw, a.tw, a.L);
+    load_table(tw, a.tw, a.L, a.sign);
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     for (int e = threadIdx.x; e < ncl * ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
@@ -337,7 +345,7 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop(RowArgs a)
     const int64_t off = n0 * a.s0 + n1 * a.L;
     const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
     int *spinv = (int *)(buf + a.CL * ls);
-    load_table(tw, a.tw, a.L);
+    load_table(tw, a.tw, a.L, a.sign);
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     __syncthreads();
     const int total = ncl * a.L;

@@ -702,24 +702,26 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const Spr
 // a multiple of 4 with an all-zero pair) and accumulates
 //     D[cell i][real j] += sum over 4 listed pairs k of  w_k(cell i) * v_k[j]
 // with one MFMA per 4 pairs: rows are the sub-block's cells, columns the 2*NCG
-// reals (re, im per coil; zero columns beyond), K the pairs.  A sub-block covers
-// on average ~half its cells per visit (a 64-cell block ~30%), and the FMAs run
-// on the matrix pipe.  The products are summed in a fixed order, so the result
-// is deterministic; batch sums are folded into Kahan-compensated registers as in
-// spread_body.  Lane layout of the 16x16x4 f32 operation: A[i][k] and B[k][j]
-// in lane i + 16k and j + 16k, D[4g + v][j] in register v of lane j + 16g.
-// Needs block-aligned tiles (multiples of 4 per axis in 3-D, of 8 in 2-D).
+// reals (re, im per coil; zero columns beyond), K the pairs.  The four lists are
+// padded to the longest so the four sub-blocks' products run interleaved.  A
+// sub-block covers on average ~half its cells per visit (a 64-cell block ~30%),
+// and the FMAs run on the matrix pipe.  The products are summed in a fixed order,
+// so the result is deterministic; batch sums are folded into Kahan-compensated
+// registers as in spread_body.  Lane layout of the 16x16x4 f32 operation:
+// A[i][k] and B[k][j] in lane i + 16k and j + 16k, D[4g + v][j] in register v of
+// lane j + 16g.  The finished tile goes through LDS ([coil][cell]) and is written
+// row by row along the last axis.  Needs block-aligned tiles (multiples of 4 per
+// axis in 3-D, of 8 in 2-D).
 // ---------------------------------------------------------------------------
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-#ifndef GN_SPREAD_UNROLL2
-#define GN_SPREAD_UNROLL2 0
-#endif
 
 inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
 {
-    return (size_t)(kSpreadBatch + 1) * a.TT * 4 + (size_t)(kSpreadBatch + 1) * 16 * 4 + (size_t)kSpreadBatch * 16 +
-           (size_t)kSpreadBatch * 64 + (size_t)waves * 4 * kSpreadBatch * 4;
+    const size_t batch = (size_t)(kSpreadBatch + 1) * a.TT * 4 + (size_t)(kSpreadBatch + 1) * 16 * 4 +
+                         (size_t)kSpreadBatch * 16 + (size_t)kSpreadBatch * 64 + (size_t)waves * 4 * kSpreadBatch * 4;
+    const size_t out = (size_t)8 * (a.TC + 1) * 8; // output staging: up to 8 coils of the tile
+    return std::max(batch, out);
 }
 
 template <int D, int NCG>
@@ -859,72 +861,73 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
             m1 = (uint32_t)s_msk[lane * 4 + 1];
             m2 = D == 3 ? (uint32_t)s_msk[lane * 4 + 2] : 1u;
         }
-        int cnt4[4];
+        int cnt[4], cmax = 0;
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
             const uint32_t bm0 = (D == 3 ? 0x3u : 0xfu) << ((D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1));
             const uint32_t bm1 = (D == 3 ? 0x3u : 0xfu) << ((D == 3 ? bq1 * 4 : bq1 * 8) + SB1 * (sb & 1));
             const bool hit = (m0 & bm0) && (m1 & bm1) && (m2 & sbm2);
             const unsigned long long hm = __ballot(hit);
-            const int cnt = __popcll(hm);
+            cnt[sb] = __popcll(hm);
+            cmax = max(cmax, (cnt[sb] + 3) & ~3);
             int *lst = s_list + (wave * 4 + sb) * B;
             const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0));
             if (hit) lst[pos] = lane;
-            cnt4[sb] = (cnt + 3) & ~3;
-            if (lane >= cnt && lane < cnt4[sb]) lst[lane] = B;
         }
+        // every list padded with the zero pair to the longest: the four sub-blocks'
+        // products then run interleaved, their LDS reads in flight together
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb)
+            if (lane >= cnt[sb] && lane < cmax) s_list[(wave * 4 + sb) * B + lane] = B;
         // the lists were written by this wave's own lanes: order the LDS accesses
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        f32x4 acc[4];
 #pragma unroll
-        for (int sb = 0; sb < 4; ++sb) {
-            const int *lst = s_list + (wave * 4 + sb) * B;
-            const int X = X0 + SB0 * (sb >> 1), Y = Y0 + SB1 * (sb & 1);
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            int t = 0;
-#if GN_SPREAD_UNROLL2
-            // two independent steps per iteration: their LDS reads overlap
-            f32x4 acc1 = {0.f, 0.f, 0.f, 0.f};
-            for (; t + 8 <= cnt4[sb]; t += 8) {
-                const int ka = lst[t + kg], kb = lst[t + 4 + kg];
-                const float *ta = s_tab + ka * TT, *tb = s_tab + kb * TT;
-                float wa = ta[X] * ta[to1 + Y], wb = tb[X] * tb[to1 + Y];
-                if (D == 3) {
-                    wa *= ta[to2 + Zc];
-                    wb *= tb[to2 + Zc];
-                }
-                const float va = s_val[ka * 16 + ri], vb = s_val[kb * 16 + ri];
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wa, va, acc, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(wb, vb, acc1, 0, 0, 0);
-            }
-#endif
-            for (; t < cnt4[sb]; t += 4) {
-                const int kk = lst[t + kg];
+        for (int sb = 0; sb < 4; ++sb) acc[sb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < cmax; t += 4) {
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) {
+                const int kk = s_list[(wave * 4 + sb) * B + t + kg];
                 const float *tab = s_tab + kk * TT;
-                float w = tab[X] * tab[to1 + Y];
+                float w = tab[X0 + SB0 * (sb >> 1)] * tab[to1 + Y0 + SB1 * (sb & 1)];
                 if (D == 3) w *= tab[to2 + Zc];
                 const float v = s_val[kk * 16 + ri];
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(w, v, acc, 0, 0, 0);
+                acc[sb] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, v, acc[sb], 0, 0, 0);
             }
-#if GN_SPREAD_UNROLL2
-            acc += acc1;
-#endif
+        }
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float y = acc[v] - K[sb][v];
+                const float y = acc[sb][v] - K[sb][v];
                 const float tsum = S[sb][v] + y;
                 K[sb][v] = (tsum - S[sb][v]) - y;
                 S[sb][v] = tsum;
             }
+    }
+    // write through LDS: every wave stores its sub-blocks' sums as [coil][cell]
+    // (a cell of padding per coil), then the workgroup writes the tile row by row
+    // along the last axis, two cells per 16-byte store where alignment allows
+    __syncthreads(); // the batch buffers are reused
+    float *s_out = lds_sp;
+    const int TC = a.TC;
+    const int CS = 2 * (TC + 1); // floats per coil
+    if (has_block && (ri >> 1) < ncg) {
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const int X = (D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1);
+            const int Y = (D == 3 ? bq1 * 4 : bq1 * 8) + SB1 * (sb & 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int cell = 4 * kg + v;
+                const int L0 = X + (D == 3 ? (cell >> 3) : (cell >> 2));
+                const int L1 = Y + (D == 3 ? ((cell >> 2) & 1) : (cell & 3));
+                const int L2 = D == 3 ? bq2 * 4 + (cell & 3) : 0;
+                s_out[(ri >> 1) * CS + ((L0 * T1 + L1) * T2 + L2) * 2 + (ri & 1)] = S[sb][v] - K[sb][v];
+            }
         }
     }
-    if (!has_block) return;
-
-    // write: lane (j + 16g) holds real j of cells 4g..4g+3 of each sub-block, which
-    // are consecutive along the last axis; lanes j and j^1 swap halves so that each
-    // holds two whole complex cells (cells 4g, 4g+1 on even lanes, 4g+2, 4g+3 on odd)
-    const int j = ri;
-    const int coil = j >> 1;
-    const bool im = j & 1;
+    __syncthreads();
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     if (item.w < 0) {
         int r = item.x;
@@ -936,50 +939,49 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
             teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
     }
-    const int nflast = (int)a.nf[D - 1];
-    // 16-byte stores need an even cell offset: even grid size and last-axis length
-    const bool wide = ((a.G | nflast) & 1) == 0;
-#pragma unroll
-    for (int sb = 0; sb < 4; ++sb) {
-        float mine[4], other[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            mine[v] = S[sb][v] - K[sb][v];
-            other[v] = __shfl_xor(mine[v], 1);
-        }
-        if (coil >= ncg) continue;
-        const int v0 = im ? 2 : 0;
-        float4 val;
-        val.x = im ? other[v0] : mine[v0];
-        val.y = im ? mine[v0] : other[v0];
-        val.z = im ? other[v0 + 1] : mine[v0 + 1];
-        val.w = im ? mine[v0 + 1] : other[v0 + 1];
-        // first of the two cells: row 4g + v0 of the sub-block
-        const int cell = 4 * kg + v0;
-        const int L0 = (D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1) + (D == 3 ? (cell >> 3) : (cell >> 2));
-        const int L1 = (D == 3 ? bq1 * 4 : bq1 * 8) + SB1 * (sb & 1) + (D == 3 ? ((cell >> 2) & 1) : (cell & 3));
-        const int L2 = D == 3 ? bq2 * 4 + (cell & 3) : 0;
-        const int cidx = (L0 * T1 + L1) * T2 + L2;
+    const int Tl = D == 3 ? T2 : T1;        // cells per row along the last axis
+    const int nrow = TC / Tl;                // rows per coil
+    const int hl = Tl >> 1;                  // cell pairs per row (Tl is even)
+    const float rhl = 1.0f / (float)hl, rnr = 1.0f / (float)nrow, rT1 = 1.0f / (float)T1;
+    const bool wide = item.w >= 0 || ((a.G | a.nf[D - 1]) & 1) == 0;
+    const int npairs = ncg * nrow * hl;
+    for (int e = tid; e < npairs; e += blockDim.x) {
+        const int rowg = fdiv_small(e, hl, rhl); // (coil, row)
+        const int h = e - rowg * hl;
+        const int coil = fdiv_small(rowg, nrow, rnr);
+        const int row = rowg - coil * nrow;
+        const int cidx = row * Tl + 2 * h;
+        const float *src = s_out + coil * CS + cidx * 2;
+        const float2 c0 = make_float2(src[0], src[1]), c1 = make_float2(src[2], src[3]);
         if (item.w >= 0) {
-            float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0 + coil) * a.TC + cidx;
-            dst[0] = make_float2(val.x, val.y);
-            dst[1] = make_float2(val.z, val.w);
+            float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0 + coil) * TC + cidx;
+            *(float4 *)dst = make_float4(c0.x, c0.y, c1.x, c1.y);
             continue;
         }
+        int L0, L1, L2 = 0;
+        if (D == 3) {
+            L0 = fdiv_small(row, T1, rT1);
+            L1 = row - L0 * T1;
+            L2 = 2 * h;
+        } else {
+            L0 = row;
+            L1 = 2 * h;
+        }
         if (L0 >= teff[0] || (D == 3 && L1 >= teff[1])) continue;
-        const int Llast = D == 3 ? L2 : L1;          // position along the last axis
-        const int tlast = teff[D - 1];
+        const int Ll = D == 3 ? L2 : L1, tl = teff[D - 1];
+        if (Ll >= tl) continue;
         int64_t gi = (int64_t)(org[0] + L0) * a.nf[1] + (org[1] + L1);
         if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
         float2 *dst = a.grid + (int64_t)(cg0 + coil) * a.G + gi;
-        if (wide && Llast + 1 < tlast) {
-            *(float4 *)dst = val;
+        if (wide && Ll + 1 < tl) {
+            *(float4 *)dst = make_float4(c0.x, c0.y, c1.x, c1.y);
         } else {
-            if (Llast < tlast) dst[0] = make_float2(val.x, val.y);
-            if (Llast + 1 < tlast) dst[1] = make_float2(val.z, val.w);
+            dst[0] = c0;
+            if (Ll + 1 < tl) dst[1] = c1;
         }
     }
 }
+
 
 template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma(SpreadArgs a)
@@ -1705,7 +1707,8 @@ bool mfma_tiles_ok(const SpreadArgs &a, int dim)
     }
     // phase 2 maps one table column per thread: 64 * waves >= TT rounded up to a power of 2
     const int ttp = a.TT <= 16 ? 16 : a.TT <= 32 ? 32 : a.TT <= 64 ? 64 : 128;
-    return a.TT <= 128 && 64 * std::min(nb, kSpreadWavesMax) >= ttp;
+    const int waves = std::min(nb, kSpreadWavesMax);
+    return a.TT <= 128 && 64 * waves >= ttp && spread_mfma_lds_bytes(a, waves) <= 160 * 1024;
 }
 
 inline size_t spread_lds_bytes(const SpreadArgs &a, int ncg)

@@ -51,12 +51,22 @@ struct LaunchList {
     int4 *d = nullptr;
 };
 
+// one batched pruned-FFT launch: work items of the instances whose pass has the same
+// length; the LDS size is the largest any of them needs
+struct FFTLaunch {
+    size_t lds = 0;
+    std::vector<int4> items;
+    int4 *d = nullptr;
+};
+
 // device argument tables of one kind
 struct Tables {
     SpreadArgs *spread = nullptr;
     GatherArgs *gather = nullptr;
     PrepostArgs *pad = nullptr;
     PrepostArgs *crop = nullptr;
+    RowArgs *rowp = nullptr; // pruned pass A (pad rows) per instance
+    RowArgs *rowc = nullptr; // pruned pass A' (crop rows) per instance
     std::vector<char> last; // host image of what was uploaded
 };
 
@@ -83,6 +93,10 @@ struct gn_batch_s {
     float2 *ybuf = nullptr;     // type-21 intermediate samples
     std::vector<int64_t> yoff;
     std::vector<FFTGroup> groups;
+    bool custom = false;               // pruned FFT passes (every grid length supported)
+    std::vector<FFTLaunch> frows;      // items {instance, line, coil chunk}
+    std::vector<FFTLaunch> fcols[2][2]; // [type 2, type 1][pass]: items {argument, column block, slab}
+    ColArgs *d_cols[2] = {nullptr, nullptr}; // [type 2, type 1]: 2 entries per instance
     rocfft_execution_info info = nullptr;
     void *work = nullptr;
     std::vector<LaunchList> spread_lists, gather_lists;
@@ -229,6 +243,7 @@ int make_groups(gn_batch_s *b)
         return fail(GN_ENOMEM, "hipMalloc failed for the batch grid buffer");
     for (int i = 0; i < b->n; ++i) b->m[i]->grid = b->grid + ((intptr_t)b->m[i]->grid / (intptr_t)sizeof(float2));
 
+    if (b->custom) return GN_OK; // pruned passes: no rocFFT plans
     rocfft_status s;
     size_t wmax = 0;
     for (auto &fg : b->groups) {
@@ -276,6 +291,79 @@ int run_ffts(gn_batch_s *b, bool forward, hipStream_t st)
     return GN_OK;
 }
 
+// Work items and column-pass arguments of the batched pruned FFT (geometry only).
+int build_fft_lists(gn_batch_s *b)
+{
+    const float sign2 = b->iflag >= 0 ? -1.0f : 1.0f;
+    std::vector<ColArgs> cols[2];
+    std::map<int, size_t> rkey;
+    std::map<int, size_t> ckey[2][2];
+    for (int t = 0; t < 2; ++t) cols[t].resize(2 * (size_t)b->n);
+    for (int i = 0; i < b->n; ++i) {
+        const gn_plan_s *p = b->m[i];
+        const RowArgs ra = make_row_args(p, sign2, 1.0f);
+        if (!rkey.count(ra.L)) {
+            rkey[ra.L] = b->frows.size();
+            b->frows.push_back(FFTLaunch());
+        }
+        FFTLaunch &rl = b->frows[rkey[ra.L]];
+        rl.lds = std::max(rl.lds, row_lds(ra.L, ra.CL)); // coils per block vary with the instance
+        const int64_t nch = (p->nc + ra.CL - 1) / ra.CL;
+        for (int64_t line = 0; line < ra.nlines; ++line)
+            for (int64_t c = 0; c < nch; ++c) rl.items.push_back(make_int4(i, (int)line, (int)c, 0));
+        for (int t = 0; t < 2; ++t) {
+            const int np = pruned_col_args(p, t == 0, t == 0 ? sign2 : -sign2, &cols[t][2 * (size_t)i]);
+            for (int k = 0; k < np; ++k) {
+                const ColArgs &a = cols[t][2 * (size_t)i + k];
+                if (!ckey[t][k].count(a.L)) {
+                    ckey[t][k][a.L] = b->fcols[t][k].size();
+                    FFTLaunch l;
+                    l.lds = col_lds(a.L, a.CB);
+                    b->fcols[t][k].push_back(l);
+                }
+                FFTLaunch &cl = b->fcols[t][k][ckey[t][k][a.L]];
+                // column blocks fastest: neighbouring blocks read neighbouring columns
+                const int64_t ncb = (a.ncols + a.CB - 1) / a.CB;
+                for (int64_t sl = 0; sl < a.nslabs; ++sl)
+                    for (int64_t cb = 0; cb < ncb; ++cb)
+                        cl.items.push_back(make_int4(2 * i + k, (int)cb, (int)sl, 0));
+            }
+        }
+    }
+    int rc;
+    for (auto &l : b->frows)
+        if ((rc = upload_list(l.items, &l.d))) return rc;
+    for (int t = 0; t < 2; ++t) {
+        for (int k = 0; k < 2; ++k)
+            for (auto &l : b->fcols[t][k])
+                if ((rc = upload_list(l.items, &l.d))) return rc;
+        if (hipMalloc((void **)&b->d_cols[t], cols[t].size() * sizeof(ColArgs)) != hipSuccess)
+            return fail(GN_ENOMEM, "batch FFT argument tables");
+        hipError_t e = hipMemcpy(b->d_cols[t], cols[t].data(), cols[t].size() * sizeof(ColArgs), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "batch FFT argument tables");
+    }
+    return GN_OK;
+}
+
+int launch_fft_cols(gn_batch_s *b, int t, hipStream_t st)
+{
+    for (int k = 0; k < 2; ++k)
+        for (auto &l : b->fcols[t][k]) {
+            hipError_t e = launch_fft_cols_batch(b->d_cols[t], l.d, (int)l.items.size(), l.lds, st);
+            if (e != hipSuccess) return hip_fail(e, "batch FFT column pass");
+        }
+    return GN_OK;
+}
+
+int launch_fft_rows(gn_batch_s *b, bool pad, const RowArgs *args, hipStream_t st)
+{
+    for (auto &l : b->frows) {
+        hipError_t e = launch_fft_rows_batch(pad, args, l.d, (int)l.items.size(), l.lds, st);
+        if (e != hipSuccess) return hip_fail(e, pad ? "batch FFT pass A" : "batch FFT pass A'");
+    }
+    return GN_OK;
+}
+
 // Host argument tables of one kind for these data pointers; uploaded when changed.
 int prepare_tables(gn_batch_s *b, int kind, const void *const *in, void *const *out, hipStream_t st)
 {
@@ -285,6 +373,8 @@ int prepare_tables(gn_batch_s *b, int kind, const void *const *in, void *const *
     std::vector<SpreadArgs> sa(n);
     std::vector<GatherArgs> ga(n);
     std::vector<PrepostArgs> pa(n), ca(n);
+    std::vector<RowArgs> rp(b->custom ? n : 0), rcp(b->custom ? n : 0);
+    const float sign2 = b->iflag >= 0 ? -1.0f : 1.0f;
     const double pi = 3.141592653589793;
     for (int i = 0; i < n; ++i) {
         gn_plan_s *p = b->m[i];
@@ -303,6 +393,12 @@ int prepare_tables(gn_batch_s *b, int kind, const void *const *in, void *const *
         ca[i] = make_prepost(p, (float)(c * extra / (double)g.G));
         ca[i].src = p->grid;
         ca[i].dst = t1 ? (float2 *)out[i] : nullptr;
+        if (b->custom) {
+            rp[i] = make_row_args(p, sign2, pa[i].scale);
+            rp[i].img_in = pa[i].src;
+            rcp[i] = make_row_args(p, -sign2, ca[i].scale);
+            rcp[i].img_out = ca[i].dst;
+        }
     }
     std::vector<char> img;
     auto put = [&](const void *ptr, size_t nb) {
@@ -313,6 +409,8 @@ int prepare_tables(gn_batch_s *b, int kind, const void *const *in, void *const *
     put(ga.data(), ga.size() * sizeof(GatherArgs));
     put(pa.data(), pa.size() * sizeof(PrepostArgs));
     put(ca.data(), ca.size() * sizeof(PrepostArgs));
+    put(rp.data(), rp.size() * sizeof(RowArgs));
+    put(rcp.data(), rcp.size() * sizeof(RowArgs));
     if (img == t.last) return GN_OK;
     hipError_t e = hipMemcpyAsync(t.spread, sa.data(), sa.size() * sizeof(SpreadArgs), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
@@ -321,6 +419,10 @@ int prepare_tables(gn_batch_s *b, int kind, const void *const *in, void *const *
         e = hipMemcpyAsync(t.pad, pa.data(), pa.size() * sizeof(PrepostArgs), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(t.crop, ca.data(), ca.size() * sizeof(PrepostArgs), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && b->custom)
+        e = hipMemcpyAsync(t.rowp, rp.data(), rp.size() * sizeof(RowArgs), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && b->custom)
+        e = hipMemcpyAsync(t.rowc, rcp.data(), rcp.size() * sizeof(RowArgs), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st); // the host vectors die with this scope
     if (e != hipSuccess) return hip_fail(e, "batch argument upload");
     t.last.swap(img);
@@ -335,8 +437,13 @@ int enqueue(gn_batch_s *b, int kind, hipStream_t st)
     hipError_t e;
     int rc;
     auto type2 = [&]() -> int {
-        if ((e = launch_pad_batch(b->dim, t.pad, b->n, b->maxG, st)) != hipSuccess) return hip_fail(e, "batch pad");
-        if ((rc = run_ffts(b, fwd_dir_type2, st))) return rc;
+        if (b->custom) {
+            if ((rc = launch_fft_rows(b, true, t.rowp, st))) return rc;
+            if ((rc = launch_fft_cols(b, 0, st))) return rc;
+        } else {
+            if ((e = launch_pad_batch(b->dim, t.pad, b->n, b->maxG, st)) != hipSuccess) return hip_fail(e, "batch pad");
+            if ((rc = run_ffts(b, fwd_dir_type2, st))) return rc;
+        }
         for (auto &l : b->gather_lists)
             if ((e = launch_gather_batch(b->dim, b->ns, l.wrap, t.gather, l.d, (int)l.items.size(), l.lds, st)) !=
                 hipSuccess)
@@ -351,6 +458,10 @@ int enqueue(gn_batch_s *b, int kind, hipStream_t st)
         if ((e = launch_spread_reduce_batch(b->dim, t.spread, b->d_reduce, (int)b->reduce_items.size(), st)) !=
             hipSuccess)
             return hip_fail(e, "batch spread reduce");
+        if (b->custom) {
+            if ((rc = launch_fft_cols(b, 1, st))) return rc;
+            return launch_fft_rows(b, false, t.rowc, st);
+        }
         if ((rc = run_ffts(b, !fwd_dir_type2, st))) return rc;
         if ((e = launch_crop_batch(b->dim, t.crop, b->n, b->maxN, st)) != hipSuccess) return hip_fail(e, "batch crop");
         return GN_OK;
@@ -408,6 +519,14 @@ void destroy(gn_batch_s *b)
         hipFree(t.gather);
         hipFree(t.pad);
         hipFree(t.crop);
+        hipFree(t.rowp);
+        hipFree(t.rowc);
+    }
+    for (auto &l : b->frows) hipFree(l.d);
+    for (int t = 0; t < 2; ++t) {
+        for (int k = 0; k < 2; ++k)
+            for (auto &l : b->fcols[t][k]) hipFree(l.d);
+        hipFree(b->d_cols[t]);
     }
     for (auto &fg : b->groups) {
         if (fg.fwd) rocfft_plan_destroy(fg.fwd);
@@ -557,7 +676,16 @@ int gn_batch_create(int dim, int n, const gn_instance *inst, int rat, int nsprea
         b->maxG = std::max(b->maxG, p->geo.G);
         b->maxN = std::max(b->maxN, p->geo.N);
     }
+    if (!rc) {
+        // pruned FFT passes when every instance's grid lengths are supported
+        b->custom = true;
+        for (auto *p : b->m)
+            for (int a = 0; a < dim; ++a) b->custom &= fft_length_supported(p->geo.nf[a]);
+        for (auto *p : b->m)
+            for (int a = 0; a < dim && b->custom && !rc; ++a) rc = fft_axis_init(&p->fft_ax[a], (int)p->geo.nf[a]);
+    }
     if (!rc) rc = make_groups(b);
+    if (!rc && b->custom) rc = build_fft_lists(b);
     if (!rc && yel > 0 && hipMalloc((void **)&b->ybuf, (size_t)yel * sizeof(float2)) != hipSuccess)
         rc = fail(GN_ENOMEM, "type-21 sample buffer");
     for (int k = 0; k < kKinds && !rc; ++k) {
@@ -565,7 +693,9 @@ int gn_batch_create(int dim, int n, const gn_instance *inst, int rat, int nsprea
         if (hipMalloc((void **)&t.spread, n * sizeof(SpreadArgs)) != hipSuccess ||
             hipMalloc((void **)&t.gather, n * sizeof(GatherArgs)) != hipSuccess ||
             hipMalloc((void **)&t.pad, n * sizeof(PrepostArgs)) != hipSuccess ||
-            hipMalloc((void **)&t.crop, n * sizeof(PrepostArgs)) != hipSuccess)
+            hipMalloc((void **)&t.crop, n * sizeof(PrepostArgs)) != hipSuccess ||
+            (b->custom && (hipMalloc((void **)&t.rowp, n * sizeof(RowArgs)) != hipSuccess ||
+                           hipMalloc((void **)&t.rowc, n * sizeof(RowArgs)) != hipSuccess)))
             rc = fail(GN_ENOMEM, "batch argument tables");
     }
     if (!rc) rc = rebin(b);
@@ -655,9 +785,17 @@ int gn_batch_info(const gn_batch *b, int *n, int *tile, int *launches)
         }
     }
     if (launches) {
-        const int nfft = (int)b->groups.size();
-        launches[0] = 1 + nfft + (int)b->gather_lists.size();
-        launches[1] = (int)b->spread_lists.size() + (b->reduce_items.empty() ? 0 : 1) + nfft + 1;
+        if (b->custom) {
+            const int rows = (int)b->frows.size();
+            const int c2 = (int)(b->fcols[0][0].size() + b->fcols[0][1].size());
+            const int c1 = (int)(b->fcols[1][0].size() + b->fcols[1][1].size());
+            launches[0] = rows + c2 + (int)b->gather_lists.size();
+            launches[1] = (int)b->spread_lists.size() + (b->reduce_items.empty() ? 0 : 1) + c1 + rows;
+        } else {
+            const int nfft = (int)b->groups.size();
+            launches[0] = 1 + nfft + (int)b->gather_lists.size();
+            launches[1] = (int)b->spread_lists.size() + (b->reduce_items.empty() ? 0 : 1) + nfft + 1;
+        }
     }
     return GN_OK;
 }

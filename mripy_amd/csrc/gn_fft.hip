@@ -46,10 +46,6 @@ constexpr int kLoadUnroll = 8;                    // global loads in flight per 
 constexpr int kFFTColThreads = GN_FFT_COL_THREADS; // column passes
 static_assert((GN_FFT_CB & (GN_FFT_CB - 1)) == 0, "column blocks index rows with shifts");
 
-struct Stages {
-    int n;
-    int radix[kFFTMaxStages];
-};
 
 __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -170,22 +166,8 @@ __device__ inline void load_table(float2 *dst, const float2 *src, int n, float s
 // (out_map) or n.
 // ---------------------------------------------------------------------------
 
-struct ColArgs {
-    float2 *data; // in place
-    int L;
-    int rows_in, rows_out;
-    int in_map, out_map; // 1: pruned position map with m = rows_in (in) / rows_out (out)
-    int64_t ncols;       // contiguous columns; also the row stride
-    int64_t slab_inner;  // slab s starts at (s / slab_inner) * stride_outer + (s % slab_inner) * stride_inner
-    int64_t stride_outer, stride_inner;
-    int CB;
-    float sign;
-    Stages st;
-    const float2 *tw;
-    const int *pinv; // FFT position -> LDS slot (digit reversal)
-};
 
-__global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
+__device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t slab)
 {
     extern __shared__ float2 lds_fft[];
     float2 *tw = lds_fft;
@@ -193,8 +175,7 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
     const int CB = a.CB;
     const int lstride = 1;        // columns adjacent
     const int estride = CB + 1;   // rows padded by one element
-    const int64_t col0 = (int64_t)blockIdx.x * CB;
-    const int64_t slab = blockIdx.y;
+    const int64_t col0 = colblk * CB;
     const int ncb = (int)min((int64_t)CB, a.ncols - col0);
     float2 *base = a.data + (slab / a.slab_inner) * a.stride_outer + (slab % a.slab_inner) * a.stride_inner + col0;
     int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
@@ -244,6 +225,17 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
     }
 }
 
+__global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a) { fft_cols_body(a, blockIdx.x, blockIdx.y); }
+
+// Batched column passes: items[b] = {argument index, column block, slab, 0}.
+__global__ __launch_bounds__(kFFTColThreads) void k_fft_cols_batch(const ColArgs *__restrict__ args,
+                                                                   const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const ColArgs &a = args[it.x]; // read in place: uniform scalar loads
+    fft_cols_body(a, it.y, it.z);
+}
+
 // ---------------------------------------------------------------------------
 // Row passes along the contiguous image axis, fused with the image read (type 2)
 // or write (type 1): deapodization, scale, zero-pad / crop, coil transpose.  A
@@ -251,35 +243,13 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
 // and a chunk of CL coils; LDS holds CL lines of L (+1) elements.
 // ---------------------------------------------------------------------------
 
-struct RowArgs {
-    const float2 *img_in; // type 2: image
-    float2 *img_out;      // type 1: image
-    float2 *grid;         // line (c, n0, n1) at c * G + n0 * s0 + n1 * L
-    int64_t G, s0;
-    int L;
-    int m;                // image points on this axis
-    int64_t nlines;       // image lines per coil (product of the other modes)
-    int64_t nc;
-    int CL;               // coils per block
-    int layout;
-    float scale;
-    float sign;
-    const float *deap_last; // deapodization along this axis (m)
-    const float *deap_a;    // along axis 0 (m0 entries)
-    const float *deap_b;    // along axis 1 (3-D only; m1 entries), null in 2-D
-    int m_b;                // m1 in 3-D (line index = n0 * m1 + n1), 1 in 2-D
-    Stages st;
-    const float2 *tw;
-    const int *pinv;
-};
 
-__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a)
+__device__ inline void fft_rows_pad_body(const RowArgs &a, int64_t line, int64_t chunk)
 {
     extern __shared__ float2 lds_fft[];
     float2 *tw = lds_fft;
     float2 *buf = lds_fft + a.L;
-    const int64_t line = blockIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.y * a.CL;
+    const int64_t c0 = chunk * a.CL;
     const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
     const int ls = a.L + 1;
     int *spinv = (int *)(buf + a.CL * ls);
@@ -332,13 +302,12 @@ __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a)
     }
 }
 
-__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop(RowArgs a)
+__device__ inline void fft_rows_crop_body(const RowArgs &a, int64_t line, int64_t chunk)
 {
     extern __shared__ float2 lds_fft[];
     float2 *tw = lds_fft;
     float2 *buf = lds_fft + a.L;
-    const int64_t line = blockIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.y * a.CL;
+    const int64_t c0 = chunk * a.CL;
     const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
     const int ls = a.L + 1;
     const int64_t n0 = line / a.m_b, n1 = line - (line / a.m_b) * a.m_b;
@@ -415,6 +384,25 @@ std::vector<int> fft_radices(int L)
     return r;
 }
 
+__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a) { fft_rows_pad_body(a, blockIdx.x, blockIdx.y); }
+__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop(RowArgs a) { fft_rows_crop_body(a, blockIdx.x, blockIdx.y); }
+
+// Batched row passes: items[b] = {instance, image line, coil chunk, 0}.
+__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad_batch(const RowArgs *__restrict__ args,
+                                                                    const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const RowArgs &a = args[it.x]; // read in place: uniform scalar loads
+    fft_rows_pad_body(a, it.y, it.z);
+}
+__global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop_batch(const RowArgs *__restrict__ args,
+                                                                     const int4 *__restrict__ items)
+{
+    const int4 it = items[blockIdx.x];
+    const RowArgs &a = args[it.x]; // read in place: uniform scalar loads
+    fft_rows_crop_body(a, it.y, it.z);
+}
+
 Stages make_stages(const FFTAxis &ax)
 {
     Stages s;
@@ -423,15 +411,6 @@ Stages make_stages(const FFTAxis &ax)
     return s;
 }
 
-// LDS: twiddles | lines | digit-reversal map
-size_t col_lds(int L, int CB)
-{
-    return (size_t)L * sizeof(float2) + (size_t)L * (CB + 1) * sizeof(float2) + (size_t)L * sizeof(int);
-}
-size_t row_lds(int L, int CL)
-{
-    return (size_t)L * sizeof(float2) + (size_t)CL * (L + 1) * sizeof(float2) + (size_t)L * sizeof(int);
-}
 
 #ifndef GN_FFT_COL_LDS_MAX
 #define GN_FFT_COL_LDS_MAX (64 * 1024)
@@ -451,9 +430,8 @@ int pick_cl(int L, int64_t nc)
     return cl;
 }
 
-hipError_t col_pass(const FFTAxis &ax, float sign, float2 *data, int rows_in, int rows_out, int in_map, int out_map,
-                    int64_t ncols, int64_t nslabs, int64_t slab_inner, int64_t stride_outer, int64_t stride_inner,
-                    hipStream_t st)
+ColArgs make_col_args(const FFTAxis &ax, float sign, float2 *data, int rows_in, int rows_out, int in_map, int out_map,
+                      int64_t ncols, int64_t nslabs, int64_t slab_inner, int64_t stride_outer, int64_t stride_inner)
 {
     ColArgs a;
     a.data = data;
@@ -466,15 +444,25 @@ hipError_t col_pass(const FFTAxis &ax, float sign, float2 *data, int rows_in, in
     a.slab_inner = slab_inner;
     a.stride_outer = stride_outer;
     a.stride_inner = stride_inner;
+    a.nslabs = nslabs;
     a.CB = pick_cb(ax.L);
     a.sign = sign;
     a.st = make_stages(ax);
     a.tw = ax.tw;
     a.pinv = ax.pinv;
-    dim3 grid((unsigned)((ncols + a.CB - 1) / a.CB), (unsigned)nslabs);
-    const size_t lds = col_lds(ax.L, a.CB);
-    if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute((const void *)k_fft_cols, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return a;
+}
+
+void allow_lds(const void *kernel, size_t lds)
+{
+    if (lds > 64 * 1024) (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+hipError_t launch_cols(const ColArgs &a, hipStream_t st)
+{
+    dim3 grid((unsigned)((a.ncols + a.CB - 1) / a.CB), (unsigned)a.nslabs);
+    const size_t lds = col_lds(a.L, a.CB);
+    allow_lds((const void *)k_fft_cols, lds);
     hipLaunchKernelGGL(k_fft_cols, grid, dim3(kFFTColThreads), lds, st, a);
     return hipGetLastError();
 }
@@ -514,6 +502,74 @@ RowArgs row_args(const gn_plan_s *p, const FFTAxis &ax, float sign, float scale)
 }
 
 } // namespace
+
+// LDS: twiddles | lines | digit-reversal map
+size_t col_lds(int L, int CB)
+{
+    return (size_t)L * sizeof(float2) + (size_t)L * (CB + 1) * sizeof(float2) + (size_t)L * sizeof(int);
+}
+size_t row_lds(int L, int CL)
+{
+    return (size_t)L * sizeof(float2) + (size_t)CL * (L + 1) * sizeof(float2) + (size_t)L * sizeof(int);
+}
+
+RowArgs make_row_args(const gn_plan_s *p, float sign, float scale)
+{
+    return row_args(p, p->fft_ax[p->geo.dim - 1], sign, scale);
+}
+
+// The column passes of a pruned transform in execution order (type 2: B (3-D), C;
+// type 1: C', B' (3-D)); returns their number.
+int pruned_col_args(const gn_plan_s *p, bool type2, float sign, ColArgs *out)
+{
+    const Geometry &g = p->geo;
+    const int64_t nc = p->nc, m0 = g.modes[0], L2 = g.nf[g.dim - 1];
+    // B / B': slabs (coil, n0), rows n1, columns k2; C / C': slabs = coils, rows n0,
+    // columns = all cells of the other axes
+    auto passB = [&](bool t2) {
+        return t2 ? make_col_args(p->fft_ax[1], sign, p->grid, (int)g.modes[1], (int)g.nf[1], 1, 0, L2, nc * m0, m0,
+                                  g.G, g.nf[1] * L2)
+                  : make_col_args(p->fft_ax[1], sign, p->grid, (int)g.nf[1], (int)g.modes[1], 0, 1, L2, nc * m0, m0,
+                                  g.G, g.nf[1] * L2);
+    };
+    auto passC = [&](bool t2) {
+        return t2 ? make_col_args(p->fft_ax[0], sign, p->grid, (int)m0, (int)g.nf[0], 1, 0, g.G / g.nf[0], nc, 1, g.G,
+                                  0)
+                  : make_col_args(p->fft_ax[0], sign, p->grid, (int)g.nf[0], (int)m0, 0, 1, g.G / g.nf[0], nc, 1, g.G,
+                                  0);
+    };
+    int k = 0;
+    if (type2) {
+        if (g.dim == 3) out[k++] = passB(true);
+        out[k++] = passC(true);
+    } else {
+        out[k++] = passC(false);
+        if (g.dim == 3) out[k++] = passB(false);
+    }
+    return k;
+}
+
+hipError_t launch_fft_cols_batch(const ColArgs *d_args, const int4 *d_items, int n, size_t lds, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    allow_lds((const void *)k_fft_cols_batch, lds);
+    hipLaunchKernelGGL(k_fft_cols_batch, dim3((unsigned)n), dim3(kFFTColThreads), lds, st, d_args, d_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_rows_batch(bool pad, const RowArgs *d_args, const int4 *d_items, int n, size_t lds,
+                                 hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (pad) {
+        allow_lds((const void *)k_fft_rows_pad_batch, lds);
+        hipLaunchKernelGGL(k_fft_rows_pad_batch, dim3((unsigned)n), dim3(kFFTThreads), lds, st, d_args, d_items);
+    } else {
+        allow_lds((const void *)k_fft_rows_crop_batch, lds);
+        hipLaunchKernelGGL(k_fft_rows_crop_batch, dim3((unsigned)n), dim3(kFFTThreads), lds, st, d_args, d_items);
+    }
+    return hipGetLastError();
+}
 
 bool fft_length_supported(int64_t L)
 {
@@ -579,32 +635,23 @@ int pruned_type2_rows(gn_plan_s *p, const float2 *image, float scale, float sign
 // Type 2, passes B (3-D) and C: the remaining axes, expanding m rows to nf.
 int pruned_type2_cols(gn_plan_s *p, float sign)
 {
-    const Geometry &g = p->geo;
-    const int64_t nc = p->nc, m0 = g.modes[0], L2 = g.nf[g.dim - 1];
-    hipError_t e;
-    if (g.dim == 3) {
-        // B: slabs (coil, n0), rows n1 (m1 -> nf1), columns k2
-        e = col_pass(p->fft_ax[1], sign, p->grid, (int)g.modes[1], (int)g.nf[1], 1, 0, L2, nc * m0, m0, g.G,
-                     g.nf[1] * L2, p->stream);
-        if (e != hipSuccess) return hip_fail(e, "FFT pass B");
+    ColArgs a[2];
+    const int n = pruned_col_args(p, true, sign, a);
+    for (int k = 0; k < n; ++k) {
+        hipError_t e = launch_cols(a[k], p->stream);
+        if (e != hipSuccess) return hip_fail(e, k + 1 < n ? "FFT pass B" : "FFT pass C");
     }
-    // C: slabs = coils, rows n0 (m0 -> nf0), columns = all cells of the other axes
-    e = col_pass(p->fft_ax[0], sign, p->grid, (int)m0, (int)g.nf[0], 1, 0, g.G / g.nf[0], nc, 1, g.G, 0, p->stream);
-    return e == hipSuccess ? GN_OK : hip_fail(e, "FFT pass C");
+    return GN_OK;
 }
 
 // Type 1, passes C' and B' (3-D): keep m of nf rows on the leading axes (in place).
 int pruned_type1_cols(gn_plan_s *p, float sign)
 {
-    const Geometry &g = p->geo;
-    const int64_t nc = p->nc, m0 = g.modes[0], L2 = g.nf[g.dim - 1];
-    hipError_t e = col_pass(p->fft_ax[0], sign, p->grid, (int)g.nf[0], (int)m0, 0, 1, g.G / g.nf[0], nc, 1, g.G, 0,
-                            p->stream);
-    if (e != hipSuccess) return hip_fail(e, "FFT pass C'");
-    if (g.dim == 3) {
-        e = col_pass(p->fft_ax[1], sign, p->grid, (int)g.nf[1], (int)g.modes[1], 0, 1, L2, nc * m0, m0, g.G,
-                     g.nf[1] * L2, p->stream);
-        if (e != hipSuccess) return hip_fail(e, "FFT pass B'");
+    ColArgs a[2];
+    const int n = pruned_col_args(p, false, sign, a);
+    for (int k = 0; k < n; ++k) {
+        hipError_t e = launch_cols(a[k], p->stream);
+        if (e != hipSuccess) return hip_fail(e, k == 0 ? "FFT pass C'" : "FFT pass B'");
     }
     return GN_OK;
 }

@@ -38,6 +38,50 @@ struct FFTAxis {
     int *pinv = nullptr;
 };
 
+// launch arguments of the pruned FFT passes (fft.hip)
+struct Stages {
+    int n;
+    int radix[kFFTMaxStages];
+};
+
+struct ColArgs {
+    float2 *data; // in place
+    int L;
+    int rows_in, rows_out;
+    int in_map, out_map; // 1: pruned position map with m = rows_in (in) / rows_out (out)
+    int64_t ncols;       // contiguous columns; also the row stride
+    int64_t slab_inner;  // slab s starts at (s / slab_inner) * stride_outer + (s % slab_inner) * stride_inner
+    int64_t stride_outer, stride_inner;
+    int64_t nslabs;      // slabs of this pass (the launch's second dimension)
+    int CB;
+    float sign;
+    Stages st;
+    const float2 *tw;
+    const int *pinv; // FFT position -> LDS slot (digit reversal)
+};
+
+struct RowArgs {
+    const float2 *img_in; // type 2: image
+    float2 *img_out;      // type 1: image
+    float2 *grid;         // line (c, n0, n1) at c * G + n0 * s0 + n1 * L
+    int64_t G, s0;
+    int L;
+    int m;                // image points on this axis
+    int64_t nlines;       // image lines per coil (product of the other modes)
+    int64_t nc;
+    int CL;               // coils per block
+    int layout;
+    float scale;
+    float sign;
+    const float *deap_last; // deapodization along this axis (m)
+    const float *deap_a;    // along axis 0 (m0 entries)
+    const float *deap_b;    // along axis 1 (3-D only; m1 entries), null in 2-D
+    int m_b;                // m1 in 3-D (line index = n0 * m1 + n1), 1 in 2-D
+    Stages st;
+    const float2 *tw;
+    const int *pinv;
+};
+
 // Sample records in tile-sorted order: start cell of the window on each axis
 // (already wrapped to [0,nf)), the sample's original index, its offset
 // t = xi/h - m in (-1,0] on each axis and its density weight.
@@ -211,6 +255,14 @@ int pruned_type2_rows(gn_plan_s *p, const float2 *image, float scale, float sign
 int pruned_type2_cols(gn_plan_s *p, float sign);
 int pruned_type1_cols(gn_plan_s *p, float sign);
 int pruned_type1_rows(gn_plan_s *p, float2 *image, float scale, float sign);
+// pieces of the pruned passes for the batched launch (batch.hip)
+size_t col_lds(int L, int CB);
+size_t row_lds(int L, int CL);
+RowArgs make_row_args(const gn_plan_s *p, float sign, float scale);
+int pruned_col_args(const gn_plan_s *p, bool type2, float sign, ColArgs *out);
+hipError_t launch_fft_cols_batch(const ColArgs *d_args, const int4 *d_items, int n, size_t lds, hipStream_t st);
+hipError_t launch_fft_rows_batch(bool pad, const RowArgs *d_args, const int4 *d_items, int n, size_t lds,
+                                 hipStream_t st);
 
 // api.hip
 int plan_create_impl(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,

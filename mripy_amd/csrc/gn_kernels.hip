@@ -688,7 +688,7 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const Spr
                                                                         const int4 *__restrict__ bitems)
 {
     const int4 bi = bitems[blockIdx.x];
-    const SpreadArgs a = args[bi.x];
+    const SpreadArgs &a = args[bi.x]; // read in place: uniform scalar loads
     spread_body<D, NCG>(a, a.items[bi.y], bi.z, blockIdx.y);
 }
 
@@ -999,7 +999,7 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma_batch(cons
                                                                              const int4 *__restrict__ bitems)
 {
     const int4 bi = bitems[blockIdx.x];
-    const SpreadArgs a = args[bi.x];
+    const SpreadArgs &a = args[bi.x]; // read in place: uniform scalar loads
     spread_mfma_body<D, NCG>(a, a.items[bi.y], bi.z, blockIdx.y);
 }
 
@@ -1061,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_spread_reduce_batch(const SpreadArgs *_
                                                              const int4 *__restrict__ bitems)
 {
     const int4 bi = bitems[blockIdx.x];
-    const SpreadArgs a = args[bi.x];
+    const SpreadArgs &a = args[bi.x]; // read in place: uniform scalar loads
     spread_reduce_body<D>(a, a.reduce_items[bi.y]);
 }
 
@@ -1178,7 +1178,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_batch(const GatherArg
                                                                  const int4 *__restrict__ bitems)
 {
     const int4 bi = bitems[blockIdx.x];
-    const GatherArgs a = args[bi.x];
+    const GatherArgs &a = args[bi.x]; // read in place: uniform scalar loads
     gather_body<D, NS, WRAP>(a, a.items[bi.y], bi.z);
 }
 
@@ -1357,7 +1357,7 @@ __global__ __launch_bounds__(256) void k_pad(PrepostArgs a, const float2 *__rest
 template <int D>
 __global__ __launch_bounds__(256) void k_pad_batch(const PrepostArgs *__restrict__ args)
 {
-    const PrepostArgs a = args[blockIdx.y];
+    const PrepostArgs &a = args[blockIdx.y]; // read in place: uniform scalar loads
     pad_body<D>(a, a.src, a.dst);
 }
 
@@ -1409,7 +1409,7 @@ __global__ __launch_bounds__(256) void k_crop(PrepostArgs a, const float2 *__res
 template <int D>
 __global__ __launch_bounds__(256) void k_crop_batch(const PrepostArgs *__restrict__ args)
 {
-    const PrepostArgs a = args[blockIdx.y];
+    const PrepostArgs &a = args[blockIdx.y]; // read in place: uniform scalar loads
     crop_body<D>(a, a.src, a.dst);
 }
 

@@ -1,8 +1,7 @@
 """GPU tests of the batched uneven launch, graph replay and autotuner (N7).
 
 Per-instance results of one batched execution must equal the plain per-plan
-results (rel-L2 <= 1e-6: the same kernels on the same work items; only the
-rocFFT batch count differs), graph replays must equal eager execution bit for bit
+results (rel-L2 <= 1e-6: the same kernels on the same work items), graph replays must equal eager execution bit for bit
 (with the inputs rewritten in place and re-pointed between replays), and the
 autotuned configuration must stay within 1e-5 of the FP64 oracle.
 """
@@ -60,7 +59,9 @@ def test_batch_equals_plain_plans_all_kinds():
         c64 = [c.astype(np.float64) for c in inst[i]["coords"]]
         assert rel(B.type2(imgs)[i], O.type2(c64, inst[i]["image"], p)) <= 1e-5
     info = B.info()
-    assert info["instances"] == 12 and info["launches_type2"] <= 12
+    # 4 grid lengths: one launch per length for the row pass and the column pass,
+    # plus the gather launch shapes; 12 plain plans need 36
+    assert info["instances"] == 12 and info["launches_type2"] <= 14
 
 
 def test_graph_replays_equal_eager():

@@ -15,11 +15,12 @@
 // (2-D: passes A and C only.)  Every pass works in place in the plan's grid buffer:
 // a block reads all rows of its (slab, column block) into LDS before it writes
 // any, and no two blocks share an address, so the partial transforms live in the
-// leading rows of the grid itself and no scratch buffer is needed.  Deapodization and scale are applied in A (type 2)
-// and A' (type 1), which also read / write the caller's coil-last or coil-major
-// image.  Each 1-D FFT runs in LDS as an in-place mixed-radix (2, 3, 4, 5)
-// decimation-in-time transform: loads scatter the input to digit-reversed slots,
-// so the stages need no reordering and the output comes out in natural order.
+// leading rows of the grid itself and no scratch buffer is needed.  Deapodization
+// and scale are applied in A (type 2) and A' (type 1), which also read / write the
+// caller's coil-last or coil-major image.  Each 1-D FFT runs in LDS as an in-place
+// mixed-radix (8, 4, 2, 3, 5) decimation-in-time transform: loads scatter the input
+// to digit-reversed slots, so the stages need no reordering and the output comes
+// out in natural order.
 // Twiddles come from an FP64-computed table of e^{-2 pi i k / L}.
 #include <hip/hip_runtime.h>
 
@@ -86,6 +87,28 @@ __device__ inline void butterfly(float2 *buf, int base, int qstride, const float
         const float2 r = make_float2(-s1 * b.y, s1 * b.x);
         buf[base + qstride] = cadd(m, r);
         buf[base + 2 * qstride] = csub(m, r);
+    } else if (R == 8) {
+        // X[m] = E[m] + w^m O[m], X[m + 4] = E[m] - w^m O[m]: E, O the 4-point DFTs of
+        // the even and odd inputs, w = e^{sign 2 pi i / 8}
+        const float2 s04 = cadd(v[0], v[4]), d04 = csub(v[0], v[4]);
+        const float2 s26 = cadd(v[2], v[6]), d26 = mul_si(csub(v[2], v[6]), sign);
+        const float2 e0 = cadd(s04, s26), e1 = cadd(d04, d26), e2 = csub(s04, s26), e3 = csub(d04, d26);
+        const float2 s15 = cadd(v[1], v[5]), d15 = csub(v[1], v[5]);
+        const float2 s37 = cadd(v[3], v[7]), d37 = mul_si(csub(v[3], v[7]), sign);
+        const float2 o0 = cadd(s15, s37), o1 = cadd(d15, d37), o2 = csub(s15, s37), o3 = csub(d15, d37);
+        const float r2 = 0.70710678118654752440f;
+        // w^1 = (1 + sign i) / sqrt 2, w^2 = sign i, w^3 = (-1 + sign i) / sqrt 2
+        const float2 t1 = make_float2(r2 * (o1.x - sign * o1.y), r2 * (o1.y + sign * o1.x));
+        const float2 t2 = mul_si(o2, sign);
+        const float2 t3 = make_float2(r2 * (-o3.x - sign * o3.y), r2 * (-o3.y + sign * o3.x));
+        buf[base] = cadd(e0, o0);
+        buf[base + 4 * qstride] = csub(e0, o0);
+        buf[base + qstride] = cadd(e1, t1);
+        buf[base + 5 * qstride] = csub(e1, t1);
+        buf[base + 2 * qstride] = cadd(e2, t2);
+        buf[base + 6 * qstride] = csub(e2, t2);
+        buf[base + 3 * qstride] = cadd(e3, t3);
+        buf[base + 7 * qstride] = csub(e3, t3);
     } else if (R == 5) {
         const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
         const float s1 = 0.95105651629515357212f * sign, s2 = 0.58778525229247312917f * sign;
@@ -141,6 +164,7 @@ __device__ inline void fft_lines(float2 *buf, int nlines, int lstride, int estri
             case 2: butterfly<2>(buf, base, q * estride, tw, twstep, t, sign); break;
             case 3: butterfly<3>(buf, base, q * estride, tw, twstep, t, sign); break;
             case 4: butterfly<4>(buf, base, q * estride, tw, twstep, t, sign); break;
+            case 8: butterfly<8>(buf, base, q * estride, tw, twstep, t, sign); break;
             default: butterfly<5>(buf, base, q * estride, tw, twstep, t, sign); break;
             }
         }
@@ -366,6 +390,10 @@ std::vector<int> fft_radices(int L)
 {
     std::vector<int> r;
     int n = L;
+    while (n % 8 == 0 && n > 1) {
+        r.push_back(8);
+        n /= 8;
+    }
     while (n % 4 == 0 && n > 1) {
         r.push_back(4);
         n /= 4;

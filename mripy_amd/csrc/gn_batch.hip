@@ -190,7 +190,7 @@ int build_lists(gn_batch_s *b)
             b->gather_lists.push_back(l);
         }
         LaunchList &gl = b->gather_lists[gkey[gk]];
-        const int gcg = (int)((p->nc + kCoilGroup - 1) / kCoilGroup);
+        const int gcg = (int)((p->nc + kGatherCoils - 1) / kGatherCoils);
         if (p->M > 0)
             for (int k = 0; k < p->rec.n_gather_items; ++k)
                 for (int c = 0; c < gcg; ++c) gl.items.push_back(make_int4(i, k, c, 0));

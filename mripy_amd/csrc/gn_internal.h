@@ -21,6 +21,12 @@ constexpr int kMaxDim = 3;
 #endif
 constexpr int kGatherThreads = GN_GATHER_THREADS; // threads per gather workgroup
 constexpr int kCoilGroup = 8;       // coils accumulated together per pass
+#ifndef GN_GATHER_COILS
+#define GN_GATHER_COILS 1
+#endif
+// coils per gather work-group, staged one at a time (one coil per work-group lets
+// other work-groups hide a region's staging latency)
+constexpr int kGatherCoils = GN_GATHER_COILS;
 #ifndef GN_SPREAD_CHUNK
 #define GN_SPREAD_CHUNK 1024
 #endif

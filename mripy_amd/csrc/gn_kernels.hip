@@ -1079,8 +1079,8 @@ __device__ inline void gather_body(const GatherArgs &a, const int4 item, const i
 {
     constexpr int W = 2 * NS;
     extern __shared__ float2 lds_gather[];
-    const int cg0 = cgi * kCoilGroup;
-    const int ncg = (int)min((int64_t)kCoilGroup, a.nc - cg0);
+    const int cg0 = cgi * kGatherCoils;
+    const int ncg = (int)min((int64_t)kGatherCoils, a.nc - cg0);
 
     int org[3] = {0, 0, 0};
     {
@@ -1169,7 +1169,8 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
 {
     const int64_t n = (int64_t)gridDim.x * gridDim.y;
     const int64_t w = xcd_swizzle(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, n);
-    gather_body<D, NS, WRAP>(a, a.items[w % gridDim.x], (int)(w / gridDim.x));
+    // coil groups fastest: a tile's coils run side by side and share its sample records in L2
+    gather_body<D, NS, WRAP>(a, a.items[w / gridDim.y], (int)(w % gridDim.y));
 }
 
 // Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0}.
@@ -1763,7 +1764,7 @@ hipError_t gather_dispatch(const GatherArgs &a, int nitems, int64_t nc, hipStrea
     const size_t lds = (size_t)a.region[0] * a.region[1] * (D == 3 ? a.region[2] : 1) * sizeof(float2);
     bool wrap = false;
     for (int ax = 0; ax < D; ++ax) wrap |= a.region[ax] >= a.nf[ax];
-    dim3 grid((unsigned)nitems, (unsigned)((nc + kCoilGroup - 1) / kCoilGroup));
+    dim3 grid((unsigned)nitems, (unsigned)((nc + kGatherCoils - 1) / kGatherCoils));
     if (wrap) {
         lds_limit(k_gather<D, NS, true>, lds);
         hipLaunchKernelGGL((k_gather<D, NS, true>), grid, dim3(kGatherThreads), lds, st, a);

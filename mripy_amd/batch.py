@@ -3,9 +3,9 @@
 The reference runs such workloads one operator call at a time
 (``NUFFT3d_cuda.forward/backward``, pics/operators_cuda_class.py:107-120), each call
 launching its own kernels and FFT plans.  ``BatchPlan`` keeps one plan per instance
-inside one C batch: every stage (pad, gather, spread, split-tile reduce, crop) is
-one launch over all instances' work items, each distinct grid size is one batched
-rocFFT execution, and with ``graph=True`` the whole sequence is replayed from a
+inside one C batch: every stage (gather, spread, split-tile reduce, and the pruned
+FFT passes fused with pad/crop, one launch per grid length) is one launch over all
+instances' work items, and with ``graph=True`` the whole sequence is replayed from a
 captured HIP graph.  Instance i's result equals what ``Plan`` computes for it.
 """
 from __future__ import annotations

@@ -177,3 +177,30 @@ def test_uneven_64_instance_workload_batched_equals_plain():
         worst1 = max(worst1, rel(ims[i], plain(ins, "type1", ys[i])))
     print("max rel vs plain: type2 %.3g type1 %.3g" % (worst2, worst1), flush=True)
     assert worst2 <= 1e-5 and worst1 <= 1e-5
+
+
+def test_batch_3d_equals_plain():
+    """3-D instances of three grid sizes: the batched pruned FFT runs passes A, B, C
+    (and C', B', A') over all instances; every kind equals the plain plans."""
+    import torch
+    from mripy_amd.batch import BatchPlan
+    print("case: 3-D batch vs plain", flush=True)
+    rng = np.random.default_rng(21)
+    inst = []
+    for modes, nc in (((16, 16, 16), 3), ((24, 20, 16), 1), ((16, 16, 16), 9), ((20, 24, 12), 5)):
+        M = 3000 + 500 * nc
+        coords = [(np.pi * (2 * rng.random(M) - 1)).astype(np.float32) for _ in range(3)]
+        img = (rng.standard_normal(modes + (nc,)) + 1j * rng.standard_normal(modes + (nc,))).astype(np.complex64)
+        inst.append({"modes": modes, "coords": coords, "ncoils": nc, "image": img})
+    B = BatchPlan(inst, rat=2, nspread=3)
+    imgs = [torch.as_tensor(i["image"]).cuda() for i in inst]
+    ys = [torch.as_tensor((rng.standard_normal((i["coords"][0].shape[0], i["ncoils"])) +
+                           1j * rng.standard_normal((i["coords"][0].shape[0], i["ncoils"]))).astype(np.complex64)).cuda()
+          for i in inst]
+    for kind, src in (("type2", imgs), ("type1", ys), ("type21", imgs)):
+        got = getattr(B, kind)(src)
+        for i, ins in enumerate(inst):
+            assert rel(got[i], plain(ins, kind, src[i])) <= 1e-6, (kind, i)
+    p = O.Params(inst[1]["modes"], rat=2, nspread=3)
+    c64 = [c.astype(np.float64) for c in inst[1]["coords"]]
+    assert rel(B.type2(imgs)[1], O.type2(c64, inst[1]["image"], p)) <= 1e-5

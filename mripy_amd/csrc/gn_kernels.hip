@@ -718,7 +718,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
 {
-    const size_t batch = (size_t)(kSpreadBatch + 1) * a.TT * 4 + (size_t)(kSpreadBatch + 1) * 16 * 4 +
+    const int ttp = a.TT <= 16 ? 16 : a.TT <= 32 ? 32 : a.TT <= 64 ? 64 : 128; // table row stride
+    const size_t batch = (size_t)(kSpreadBatch + 1) * ttp * 4 + (size_t)(kSpreadBatch + 1) * 16 * 4 +
                          (size_t)kSpreadBatch * 16 + (size_t)kSpreadBatch * 64 + (size_t)waves * 4 * kSpreadBatch * 4;
     const size_t out = (size_t)8 * (a.TC + 1) * 8; // output staging: up to 8 coils of the tile
     return std::max(batch, out);
@@ -729,12 +730,15 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
 {
     constexpr int B = kSpreadBatch;
     extern __shared__ float lds_sp[];
-    // LDS: weight tables [B + 1][TT] | reals [B + 1][16] | masks [B][4] | per-axis window
+    // LDS: weight tables [B + 1][ttp] | reals [B + 1][16] | masks [B][4] | per-axis window
     //      parameters [B][4] {l, c, ns - j, t} | per-wave pair lists [waves][4][B];
     //      row B of the tables and reals is the all-zero pair
     const int TT = a.TT;
+    // table rows padded to a power of two: a pair's row starts at kk << lgt
+    const int ttp = TT <= 16 ? 16 : TT <= 32 ? 32 : TT <= 64 ? 64 : 128;
+    const int lgt = __builtin_ctz(ttp);
     float *s_tab = lds_sp;
-    float *s_val = s_tab + (B + 1) * TT;
+    float *s_val = s_tab + (B + 1) * ttp;
     int *s_msk = (int *)(s_val + (B + 1) * 16);
     int4 *s_par = (int4 *)(s_msk + B * 4);
     int *s_list = (int *)(s_par + B * 4);
@@ -768,9 +772,17 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
     const int X0 = (D == 3 ? bq0 * 4 : bq0 * 8) + (D == 3 ? (ri >> 3) : (ri >> 2));
     const int Y0 = (D == 3 ? bq1 * 4 : bq1 * 8) + (D == 3 ? ((ri >> 2) & 1) : (ri & 3));
     const int Zc = D == 3 ? bq2 * 4 + (ri & 3) : 0;
+    // this lane's table columns per sub-block (axes 0 and 1) and on axis 2
+    int cx[4], cy[4];
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+        cx[sb] = X0 + SB0 * (sb >> 1);
+        cy[sb] = to1 + Y0 + SB1 * (sb & 1);
+    }
+    const int cz = to2 + Zc;
     const uint32_t sbm2 = D == 3 ? (0xfu << (bq2 * 4)) : 1u;
 
-    if (tid < TT) s_tab[B * TT + tid] = 0.f;
+    if (tid < ttp) s_tab[B * ttp + tid] = 0.f;
     if (tid < 16) s_val[B * 16 + tid] = 0.f;
 
     float S[4][4], K[4][4];
@@ -782,7 +794,6 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
     // phase-2 entry of this thread: table column e_i (axis e_ax, cell e_L) of pairs
     // e_k0, e_k0 + e_kstep, ...; columns are padded to a power of two (the workgroup
     // has at least 64 threads and TT <= 96, so at least one pair per step)
-    const int ttp = TT <= 16 ? 16 : TT <= 32 ? 32 : TT <= 64 ? 64 : 128;
     const int e_i = tid & (ttp - 1);
     const int e_k0 = tid / ttp;
     const int e_kstep = max(1, (int)blockDim.x / ttp);
@@ -849,7 +860,7 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
                     const float u = __int_as_float(par.w) + (float)(par.z - rel);
                     w = __builtin_amdgcn_exp2f(-e_coef * u * u);
                 }
-                s_tab[k * TT + e_i] = w;
+                s_tab[(k << lgt) + e_i] = w;
             }
         }
         __syncthreads();
@@ -884,14 +895,15 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
         f32x4 acc[4];
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) acc[sb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int *lst0 = s_list + wave * 4 * B + kg;
         for (int t = 0; t < cmax; t += 4) {
 #pragma unroll
             for (int sb = 0; sb < 4; ++sb) {
-                const int kk = s_list[(wave * 4 + sb) * B + t + kg];
-                const float *tab = s_tab + kk * TT;
-                float w = tab[X0 + SB0 * (sb >> 1)] * tab[to1 + Y0 + SB1 * (sb & 1)];
-                if (D == 3) w *= tab[to2 + Zc];
-                const float v = s_val[kk * 16 + ri];
+                const int kk = lst0[sb * B + t];
+                const float *tab = s_tab + (kk << lgt);
+                float w = tab[cx[sb]] * tab[cy[sb]];
+                if (D == 3) w *= tab[cz];
+                const float v = s_val[(kk << 4) + ri];
                 acc[sb] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, v, acc[sb], 0, 0, 0);
             }
         }

@@ -1530,7 +1530,7 @@ void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
         tile[1] = 4;
         tile[2] = 8;
         gtile[0] = 8;
-        gtile[1] = 8;
+        gtile[1] = 16;
         gtile[2] = 16;
     } else {
         tile[0] = 8;

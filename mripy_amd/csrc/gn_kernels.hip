@@ -682,14 +682,17 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
     spread_body<D, NCG>(a, a.items[bx], (int)(yz % gridDim.y), (int)(yz / gridDim.y));
 }
 
-// Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0}.
+// Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0},
+// taken in the XCD-aware order of the plain launch.
 template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const SpreadArgs *__restrict__ args,
                                                                         const int4 *__restrict__ bitems)
 {
-    const int4 bi = bitems[blockIdx.x];
+    const int64_t n = (int64_t)gridDim.x * gridDim.y;
+    const int64_t w = xcd_swizzle(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, n);
+    const int4 bi = bitems[w % gridDim.x];
     const SpreadArgs &a = args[bi.x]; // read in place: uniform scalar loads
-    spread_body<D, NCG>(a, a.items[bi.y], bi.z, blockIdx.y);
+    spread_body<D, NCG>(a, a.items[bi.y], bi.z, (int)(w / gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1010,9 +1013,11 @@ template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma_batch(const SpreadArgs *__restrict__ args,
                                                                              const int4 *__restrict__ bitems)
 {
-    const int4 bi = bitems[blockIdx.x];
+    const int64_t n = (int64_t)gridDim.x * gridDim.y;
+    const int64_t w = xcd_swizzle(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, n);
+    const int4 bi = bitems[w % gridDim.x];
     const SpreadArgs &a = args[bi.x]; // read in place: uniform scalar loads
-    spread_mfma_body<D, NCG>(a, a.items[bi.y], bi.z, blockIdx.y);
+    spread_mfma_body<D, NCG>(a, a.items[bi.y], bi.z, (int)(w / gridDim.x));
 }
 
 // Kahan-compensated sum of the partial tiles of each split tile, in slice order.
@@ -1189,12 +1194,13 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
     gather_body<D, NS, WRAP>(a, a.items[w / gridDim.y], (int)(w % gridDim.y));
 }
 
-// Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0}.
+// Batched launch over many plans: bitems[b] = {instance, work item, coil group, 0},
+// taken in the XCD-aware order of the plain launch.
 template <int D, int NS, bool WRAP>
 __global__ __launch_bounds__(kGatherThreads) void k_gather_batch(const GatherArgs *__restrict__ args,
                                                                  const int4 *__restrict__ bitems)
 {
-    const int4 bi = bitems[blockIdx.x];
+    const int4 bi = bitems[xcd_swizzle(blockIdx.x, gridDim.x)];
     const GatherArgs &a = args[bi.x]; // read in place: uniform scalar loads
     gather_body<D, NS, WRAP>(a, a.items[bi.y], bi.z);
 }

@@ -204,3 +204,27 @@ def test_batch_3d_equals_plain():
     p = O.Params(inst[1]["modes"], rat=2, nspread=3)
     c64 = [c.astype(np.float64) for c in inst[1]["coords"]]
     assert rel(B.type2(imgs)[1], O.type2(c64, inst[1]["image"], p)) <= 1e-5
+
+
+def test_batch_mixes_matrix_core_and_vector_spreads():
+    """A tiny instance (grid 12 x 16: its spread tiles shrink to fit and are not
+    block-aligned) runs the vector spread inside the same batch as matrix-core ones."""
+    import torch
+    from mripy_amd.batch import BatchPlan
+    print("case: batch with a vector-spread instance", flush=True)
+    rng = np.random.default_rng(23)
+    inst = []
+    for modes, nc in (((6, 8), 2), ((32, 40), 3), ((6, 8), 9), ((24, 24), 1)):
+        M = 400 + 100 * nc
+        coords = [(np.pi * (2 * rng.random(M) - 1)).astype(np.float32) for _ in range(2)]
+        inst.append({"modes": modes, "coords": coords, "ncoils": nc})
+    B = BatchPlan(inst, rat=2, nspread=3)
+    ys = [torch.as_tensor((rng.standard_normal((i["coords"][0].shape[0], i["ncoils"])) +
+                           1j * rng.standard_normal((i["coords"][0].shape[0], i["ncoils"]))).astype(np.complex64)).cuda()
+          for i in inst]
+    got = B.type1(ys)
+    for i, ins in enumerate(inst):
+        assert rel(got[i], plain(ins, "type1", ys[i])) <= 1e-6, i
+    p = O.Params(inst[0]["modes"], rat=2, nspread=3)
+    c64 = [c.astype(np.float64) for c in inst[0]["coords"]]
+    assert rel(got[0], O.type1(c64, ys[0].cpu().numpy(), p)) <= 1e-5

@@ -17,7 +17,7 @@ constexpr int kMaxDim = 3;
 #define GN_GATHER_THREADS 512
 #endif
 #ifndef GN_GATHER_CHUNK
-#define GN_GATHER_CHUNK 256
+#define GN_GATHER_CHUNK 512
 #endif
 constexpr int kGatherThreads = GN_GATHER_THREADS; // threads per gather workgroup
 constexpr int kCoilGroup = 8;       // coils accumulated together per pass

@@ -714,7 +714,8 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const Spr
 // A[i][k] and B[k][j] in lane i + 16k and j + 16k, D[4g + v][j] in register v of
 // lane j + 16g.  The finished tile goes through LDS ([coil][cell]) and is written
 // row by row along the last axis.  Needs block-aligned tiles (multiples of 4 per
-// axis in 3-D, of 8 in 2-D).
+// axis in 3-D, of 8 in 2-D) of at most kSpreadWavesMax blocks: one workgroup per
+// tile and coil group.
 // ---------------------------------------------------------------------------
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -923,11 +924,14 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
     // write through LDS: every wave stores its sub-blocks' sums as [coil][cell]
     // (a cell of padding per coil), then the workgroup writes the tile row by row
     // along the last axis, two cells per 16-byte store where alignment allows
-    __syncthreads(); // the batch buffers are reused
+    // an item without pairs (a tile no window reaches) writes zeros, skipping the
+    // staging; the condition is the same for the whole workgroup
+    const bool empty = item.y >= item.z;
     float *s_out = lds_sp;
     const int TC = a.TC;
     const int CS = 2 * (TC + 1); // floats per coil
-    if (has_block && (ri >> 1) < ncg) {
+    if (!empty) __syncthreads(); // the batch buffers are reused
+    if (!empty && has_block && (ri >> 1) < ncg) {
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
             const int X = (D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1);
@@ -942,7 +946,7 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
             }
         }
     }
-    __syncthreads();
+    if (!empty) __syncthreads();
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     if (item.w < 0) {
         int r = item.x;
@@ -966,8 +970,12 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
         const int coil = fdiv_small(rowg, nrow, rnr);
         const int row = rowg - coil * nrow;
         const int cidx = row * Tl + 2 * h;
-        const float *src = s_out + coil * CS + cidx * 2;
-        const float2 c0 = make_float2(src[0], src[1]), c1 = make_float2(src[2], src[3]);
+        float2 c0 = make_float2(0.f, 0.f), c1 = c0;
+        if (!empty) {
+            const float *src = s_out + coil * CS + cidx * 2;
+            c0 = make_float2(src[0], src[1]);
+            c1 = make_float2(src[2], src[3]);
+        }
         if (item.w >= 0) {
             float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0 + coil) * TC + cidx;
             *(float4 *)dst = make_float4(c0.x, c0.y, c1.x, c1.y);
@@ -1718,7 +1726,8 @@ void lds_limit(K kernel, size_t lds)
 #define GN_SPREAD_MFMA 1
 #endif
 // The matrix-core spread covers whole 64-cell blocks: tiles must be multiples of the
-// block (4 per axis in 3-D, 8 in 2-D).
+// block (4 per axis in 3-D, 8 in 2-D) of at most kSpreadWavesMax blocks, so that one
+// workgroup holds the whole tile.
 bool mfma_tiles_ok(const SpreadArgs &a, int dim)
 {
     if (!GN_SPREAD_MFMA) return false;
@@ -1731,7 +1740,8 @@ bool mfma_tiles_ok(const SpreadArgs &a, int dim)
     // phase 2 maps one table column per thread: 64 * waves >= TT rounded up to a power of 2
     const int ttp = a.TT <= 16 ? 16 : a.TT <= 32 ? 32 : a.TT <= 64 ? 64 : 128;
     const int waves = std::min(nb, kSpreadWavesMax);
-    return a.TT <= 128 && 64 * waves >= ttp && spread_mfma_lds_bytes(a, waves) <= 160 * 1024;
+    // one workgroup per (tile, coil group): it writes the whole tile from its LDS
+    return nb <= kSpreadWavesMax && a.TT <= 128 && 64 * waves >= ttp && spread_mfma_lds_bytes(a, waves) <= 160 * 1024;
 }
 
 inline size_t spread_lds_bytes(const SpreadArgs &a, int ncg)

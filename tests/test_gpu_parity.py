@@ -152,3 +152,20 @@ def test_dense_center_split_tiles_3d():
     c = (rng.standard_normal((M, nc)) + 1j * rng.standard_normal((M, nc))).astype(np.complex64)
     assert rel(P.type1(c), C.type1(coords, c, p)) <= TOL
     assert rel(P.type2(img), C.type2(coords, img, p)) <= TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tiles_larger_than_one_workgroup(dim):
+    """Spread tiles of more than 8 blocks (32x32 in 2-D, 8x8x16 in 3-D) span several
+    workgroups and run the vector spread; they match the oracle like the defaults."""
+    print("case: large spread tiles", dim, flush=True)
+    rng = np.random.default_rng(30 + dim)
+    modes = (48, 40) if dim == 2 else (24, 20, 32)
+    M, nc = (8000, 3) if dim == 2 else (12000, 2)
+    coords = _rand_coords(rng, M, dim)
+    p = O.Params(modes, rat=2, nspread=3)
+    P = _plan(modes, nc, rat=2, nspread=3)
+    P.set_tiles((32, 32) if dim == 2 else (8, 8, 16))
+    P.set_points(coords)
+    c = (rng.standard_normal((M, nc)) + 1j * rng.standard_normal((M, nc))).astype(np.complex64)
+    assert rel(P.type1(c), C.type1(coords, c, p)) <= TOL

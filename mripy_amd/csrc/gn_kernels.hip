@@ -759,14 +759,16 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
     const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
     const int bidx = bz * nwaves + wave;
     const bool has_block = bidx < nb;
+    // block coordinates (bidx < 2^16: exact float-reciprocal division)
     int bq0, bq1, bq2 = 0;
     if (D == 3) {
-        bq2 = bidx % a.nblk[2];
-        bq1 = (bidx / a.nblk[2]) % a.nblk[1];
-        bq0 = bidx / (a.nblk[2] * a.nblk[1]);
+        const int q2 = fdiv_small(bidx, a.nblk[2], 1.0f / (float)a.nblk[2]);
+        bq2 = bidx - q2 * a.nblk[2];
+        bq0 = fdiv_small(q2, a.nblk[1], 1.0f / (float)a.nblk[1]);
+        bq1 = q2 - bq0 * a.nblk[1];
     } else {
-        bq1 = bidx % a.nblk[1];
-        bq0 = bidx / a.nblk[1];
+        bq0 = fdiv_small(bidx, a.nblk[1], 1.0f / (float)a.nblk[1]);
+        bq1 = bidx - bq0 * a.nblk[1];
     }
     // this lane's row (cell) within a sub-block and its pair slot k = lane / 16
     const int ri = lane & 15, kg = lane >> 4;
@@ -964,10 +966,12 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
     const float rhl = 1.0f / (float)hl, rnr = 1.0f / (float)nrow, rT1 = 1.0f / (float)T1;
     const bool wide = item.w >= 0 || ((a.G | a.nf[D - 1]) & 1) == 0;
     const int npairs = ncg * nrow * hl;
+    const bool pow2 = ((hl & (hl - 1)) | (nrow & (nrow - 1))) == 0;
+    const int lgh = __builtin_ctz(hl), lgr = __builtin_ctz(nrow);
     for (int e = tid; e < npairs; e += blockDim.x) {
-        const int rowg = fdiv_small(e, hl, rhl); // (coil, row)
+        const int rowg = pow2 ? (e >> lgh) : fdiv_small(e, hl, rhl); // (coil, row)
         const int h = e - rowg * hl;
-        const int coil = fdiv_small(rowg, nrow, rnr);
+        const int coil = pow2 ? (rowg >> lgr) : fdiv_small(rowg, nrow, rnr);
         const int row = rowg - coil * nrow;
         const int cidx = row * Tl + 2 * h;
         float2 c0 = make_float2(0.f, 0.f), c1 = c0;

@@ -11,11 +11,13 @@
  * instance i, exactly what gn_execute on a plan of instance i computes.
  *
  * A batch owns one plan per instance (records binned at creation) and one
- * oversampled-grid buffer in which instances of equal grid size are adjacent,
- * so each distinct grid size is one batched rocFFT execution.  Each stage
- * (pad, gather, spread, split-tile reduce, crop) is one kernel launch over the
- * work items of all instances (a few launches when instances differ in coil
- * count class or tile shape).
+ * oversampled-grid buffer in which instances of equal grid size are adjacent.
+ * Each stage (gather, spread, split-tile reduce, and the pruned FFT passes
+ * fused with pad / crop) is one kernel launch over the work items of all
+ * instances (a few launches when instances differ in grid length, coil-count
+ * class or tile shape).  When a grid length is not a product of 2, 3 and 5 up
+ * to 2048, pad, one batched rocFFT execution per grid size and crop are used
+ * instead of the pruned passes.
  */
 #ifndef GAUSSNUFFT_BATCH_H
 #define GAUSSNUFFT_BATCH_H

@@ -1063,14 +1063,25 @@ __device__ inline void spread_reduce_body(const SpreadArgs &a, const int4 it)
         }
         if (x0 >= teff[0] || x1 >= teff[1] || (D == 3 && x2 >= teff[2])) continue;
         float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
-        for (int sl = 0; sl < it.z; ++sl) {
-            const float2 v = a.partial[((int64_t)(it.y + sl) * a.nc + coil) * TC + cidx];
-            const float yr = v.x - cr, yi = v.y - ci;
-            const float tr = sr + yr, ti = si + yi;
-            cr = (tr - sr) - yr;
-            ci = (ti - si) - yi;
-            sr = tr;
-            si = ti;
+        const float2 *src = a.partial + (int64_t)it.y * a.nc * TC + coil * TC + cidx;
+        const int64_t sstride = a.nc * TC;
+        // slices in groups of kStageUnroll: the loads are issued together, the
+        // compensated sum still runs in slice order
+        for (int s0 = 0; s0 < it.z; s0 += kStageUnroll) {
+            float2 v[kStageUnroll];
+#pragma unroll
+            for (int u = 0; u < kStageUnroll; ++u)
+                v[u] = s0 + u < it.z ? src[(int64_t)(s0 + u) * sstride] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < kStageUnroll; ++u) {
+                if (s0 + u >= it.z) break;
+                const float yr = v[u].x - cr, yi = v[u].y - ci;
+                const float tr = sr + yr, ti = si + yi;
+                cr = (tr - sr) - yr;
+                ci = (ti - si) - yi;
+                sr = tr;
+                si = ti;
+            }
         }
         int64_t gi = (int64_t)(org[0] + x0) * a.nf[1] + (org[1] + x1);
         if (D == 3) gi = gi * a.nf[2] + (org[2] + x2);

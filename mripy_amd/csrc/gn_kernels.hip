@@ -1032,7 +1032,9 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma_batch(cons
     spread_mfma_body<D, NCG>(a, a.items[bi.y], bi.z, (int)(w / gridDim.x));
 }
 
-// Kahan-compensated sum of the partial tiles of each split tile, in slice order.
+// Kahan-compensated sum of the partial tiles of each split tile, in slice order;
+// one thread per (coil, cell) of a tile (kReduceThreads covers 8 coils of 128 cells).
+constexpr int kReduceThreads = 1024;
 template <int D>
 __device__ inline void spread_reduce_body(const SpreadArgs &a, const int4 it)
 {
@@ -1090,14 +1092,14 @@ __device__ inline void spread_reduce_body(const SpreadArgs &a, const int4 it)
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_spread_reduce(SpreadArgs a, const int4 *__restrict__ ritems)
+__global__ __launch_bounds__(kReduceThreads) void k_spread_reduce(SpreadArgs a, const int4 *__restrict__ ritems)
 {
     spread_reduce_body<D>(a, ritems[blockIdx.x]);
 }
 
 // bitems[b] = {instance, reduce item}
 template <int D>
-__global__ __launch_bounds__(256) void k_spread_reduce_batch(const SpreadArgs *__restrict__ args,
+__global__ __launch_bounds__(kReduceThreads) void k_spread_reduce_batch(const SpreadArgs *__restrict__ args,
                                                              const int4 *__restrict__ bitems)
 {
     const int4 bi = bitems[blockIdx.x];
@@ -1799,7 +1801,7 @@ hipError_t spread_dispatch(const SpreadArgs &a, int nitems, int nritems, const i
     else
         e = spread_launch<D, 1>(a, nitems, nc, st);
     if (e == hipSuccess && nritems > 0) {
-        hipLaunchKernelGGL(k_spread_reduce<D>, dim3((unsigned)nritems), dim3(256), 0, st, a, ritems);
+        hipLaunchKernelGGL(k_spread_reduce<D>, dim3((unsigned)nritems), dim3(kReduceThreads), 0, st, a, ritems);
         e = hipGetLastError();
     }
     return e;
@@ -2133,9 +2135,9 @@ hipError_t launch_spread_reduce_batch(int dim, const SpreadArgs *d_args, const i
 {
     if (n <= 0) return hipSuccess;
     if (dim == 3)
-        hipLaunchKernelGGL(k_spread_reduce_batch<3>, dim3((unsigned)n), dim3(256), 0, st, d_args, d_items);
+        hipLaunchKernelGGL(k_spread_reduce_batch<3>, dim3((unsigned)n), dim3(kReduceThreads), 0, st, d_args, d_items);
     else
-        hipLaunchKernelGGL(k_spread_reduce_batch<2>, dim3((unsigned)n), dim3(256), 0, st, d_args, d_items);
+        hipLaunchKernelGGL(k_spread_reduce_batch<2>, dim3((unsigned)n), dim3(kReduceThreads), 0, st, d_args, d_items);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ constexpr int kCoilGroup = 8;       // coils accumulated together per pass
 // other work-groups hide a region's staging latency)
 constexpr int kGatherCoils = GN_GATHER_COILS;
 #ifndef GN_SPREAD_CHUNK
-#define GN_SPREAD_CHUNK 1024
+#define GN_SPREAD_CHUNK 512
 #endif
 constexpr int kSpreadChunk = GN_SPREAD_CHUNK; // max (tile, sample) pairs per spread work item
 constexpr int kGatherChunk = GN_GATHER_CHUNK;  // max samples per gather work item

@@ -87,12 +87,13 @@ __device__ inline void stage_region(float2 *lds, const float2 *__restrict__ g, c
     }
 }
 
-__device__ inline int64_t xcd_swizzle(int64_t lin, int64_t n)
+__device__ inline uint32_t xcd_swizzle(uint32_t lin, uint32_t n)
 {
-    const int64_t round = kXCDs * kXCDRun;
+    // 32-bit: every launch of these kernels has fewer than 2^31 blocks (checked on the host)
+    constexpr uint32_t run = (uint32_t)kXCDRun, round = (uint32_t)kXCDs * run;
     if (lin >= (n / round) * round) return lin;
-    const int64_t x = lin % kXCDs, k = lin / kXCDs;
-    return ((k / kXCDRun) * kXCDs + x) * kXCDRun + k % kXCDRun;
+    const uint32_t x = lin % kXCDs, k = lin / kXCDs;
+    return ((k / run) * kXCDs + x) * run + k % run;
 }
 
 // ---------------------------------------------------------------------------
@@ -674,11 +675,11 @@ __device__ inline void spread_body(const SpreadArgs &a, const int4 item, const i
 template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread(SpreadArgs a)
 {
-    const int64_t n = (int64_t)gridDim.x * gridDim.y * gridDim.z;
-    const int64_t lin = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-    const int64_t w = xcd_swizzle(lin, n);
+    const uint32_t n = gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const uint32_t w = xcd_swizzle(lin, n);
     const int bx = (int)(w % gridDim.x);
-    const int64_t yz = w / gridDim.x;
+    const uint32_t yz = w / gridDim.x;
     spread_body<D, NCG>(a, a.items[bx], (int)(yz % gridDim.y), (int)(yz / gridDim.y));
 }
 
@@ -688,8 +689,8 @@ template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const SpreadArgs *__restrict__ args,
                                                                         const int4 *__restrict__ bitems)
 {
-    const int64_t n = (int64_t)gridDim.x * gridDim.y;
-    const int64_t w = xcd_swizzle(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, n);
+    const uint32_t n = gridDim.x * gridDim.y;
+    const uint32_t w = xcd_swizzle(blockIdx.x + gridDim.x * blockIdx.y, n);
     const int4 bi = bitems[w % gridDim.x];
     const SpreadArgs &a = args[bi.x]; // read in place: uniform scalar loads
     spread_body<D, NCG>(a, a.items[bi.y], bi.z, (int)(w / gridDim.x));
@@ -1013,11 +1014,11 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
 template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma(SpreadArgs a)
 {
-    const int64_t n = (int64_t)gridDim.x * gridDim.y * gridDim.z;
-    const int64_t lin = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-    const int64_t w = xcd_swizzle(lin, n);
+    const uint32_t n = gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const uint32_t w = xcd_swizzle(lin, n);
     const int bx = (int)(w % gridDim.x);
-    const int64_t yz = w / gridDim.x;
+    const uint32_t yz = w / gridDim.x;
     spread_mfma_body<D, NCG>(a, a.items[bx], (int)(yz % gridDim.y), (int)(yz / gridDim.y));
 }
 
@@ -1025,8 +1026,8 @@ template <int D, int NCG>
 __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma_batch(const SpreadArgs *__restrict__ args,
                                                                              const int4 *__restrict__ bitems)
 {
-    const int64_t n = (int64_t)gridDim.x * gridDim.y;
-    const int64_t w = xcd_swizzle(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, n);
+    const uint32_t n = gridDim.x * gridDim.y;
+    const uint32_t w = xcd_swizzle(blockIdx.x + gridDim.x * blockIdx.y, n);
     const int4 bi = bitems[w % gridDim.x];
     const SpreadArgs &a = args[bi.x]; // read in place: uniform scalar loads
     spread_mfma_body<D, NCG>(a, a.items[bi.y], bi.z, (int)(w / gridDim.x));
@@ -1213,8 +1214,8 @@ __device__ inline void gather_body(const GatherArgs &a, const int4 item, const i
 template <int D, int NS, bool WRAP>
 __global__ __launch_bounds__(kGatherThreads) void k_gather(GatherArgs a)
 {
-    const int64_t n = (int64_t)gridDim.x * gridDim.y;
-    const int64_t w = xcd_swizzle(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, n);
+    const uint32_t n = gridDim.x * gridDim.y;
+    const uint32_t w = xcd_swizzle(blockIdx.x + gridDim.x * blockIdx.y, n);
     // coil groups fastest: a tile's coils run side by side and share its sample records in L2
     gather_body<D, NS, WRAP>(a, a.items[w / gridDim.y], (int)(w % gridDim.y));
 }
@@ -1767,6 +1768,9 @@ inline size_t spread_lds_bytes(const SpreadArgs &a, int ncg)
            (size_t)kSpreadBatch * 32;
 }
 
+// the dispatch-order arithmetic of the spread and gather kernels is 32-bit
+inline bool grid_ok(dim3 g) { return (uint64_t)g.x * g.y * g.z < (1ull << 31); }
+
 template <int D, int NCG>
 hipError_t spread_launch(const SpreadArgs &a, int nitems, int64_t nc, hipStream_t st)
 {
@@ -1775,6 +1779,7 @@ hipError_t spread_launch(const SpreadArgs &a, int nitems, int64_t nc, hipStream_
     const int waves = std::min(nb, kSpreadWavesMax);
     const int nbg = (nb + waves - 1) / waves;
     const dim3 grid((unsigned)nitems, (unsigned)ng, (unsigned)nbg);
+    if (!grid_ok(grid)) return hipErrorInvalidConfiguration;
     if (mfma_tiles_ok(a, D)) {
         const size_t lds = spread_mfma_lds_bytes(a, waves);
         lds_limit(k_spread_mfma<D, NCG>, lds);
@@ -1814,6 +1819,7 @@ hipError_t gather_dispatch(const GatherArgs &a, int nitems, int64_t nc, hipStrea
     bool wrap = false;
     for (int ax = 0; ax < D; ++ax) wrap |= a.region[ax] >= a.nf[ax];
     dim3 grid((unsigned)nitems, (unsigned)((nc + kGatherCoils - 1) / kGatherCoils));
+    if (!grid_ok(grid)) return hipErrorInvalidConfiguration;
     if (wrap) {
         lds_limit(k_gather<D, NS, true>, lds);
         hipLaunchKernelGGL((k_gather<D, NS, true>), grid, dim3(kGatherThreads), lds, st, a);
@@ -2080,6 +2086,7 @@ template <int D, int NCG>
 hipError_t spread_batch_launch(bool mfma, const SpreadArgs *d_args, const int4 *d_items, int n, int waves, int nbg,
                                size_t lds, hipStream_t st)
 {
+    if (!grid_ok(dim3((unsigned)n, (unsigned)nbg))) return hipErrorInvalidConfiguration;
     if (mfma) {
         lds_limit(k_spread_mfma_batch<D, NCG>, lds);
         hipLaunchKernelGGL((k_spread_mfma_batch<D, NCG>), dim3((unsigned)n, (unsigned)nbg), dim3(64 * waves), lds, st,

@@ -201,7 +201,9 @@ __device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t s
     const int estride = CB + 1;   // rows padded by one element
     const int64_t col0 = colblk * CB;
     const int ncb = (int)min((int64_t)CB, a.ncols - col0);
-    float2 *base = a.data + (slab / a.slab_inner) * a.stride_outer + (slab % a.slab_inner) * a.stride_inner + col0;
+    // slab and slab_inner are below 2^31: 32-bit division
+    const uint32_t so = (uint32_t)slab / (uint32_t)a.slab_inner, si = (uint32_t)slab - so * (uint32_t)a.slab_inner;
+    float2 *base = a.data + so * a.stride_outer + si * a.stride_inner + col0;
     int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
     load_table(tw, a.tw, a.L, a.sign);
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
@@ -281,7 +283,7 @@ __device__ inline void fft_rows_pad_body(const RowArgs &a, int64_t line, int64_t
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     for (int e = threadIdx.x; e < ncl * ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
-    const int64_t n0 = line / a.m_b, n1 = line - (line / a.m_b) * a.m_b;
+    const int64_t n0 = (uint32_t)line / (uint32_t)a.m_b, n1 = line - n0 * a.m_b; // line < 2^31
     const float d01 = a.scale * a.deap_a[n0] * (a.deap_b ? a.deap_b[n1] : 1.0f);
     const int half = a.m / 2;
     const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
@@ -334,7 +336,7 @@ __device__ inline void fft_rows_crop_body(const RowArgs &a, int64_t line, int64_
     const int64_t c0 = chunk * a.CL;
     const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
     const int ls = a.L + 1;
-    const int64_t n0 = line / a.m_b, n1 = line - (line / a.m_b) * a.m_b;
+    const int64_t n0 = (uint32_t)line / (uint32_t)a.m_b, n1 = line - n0 * a.m_b; // line < 2^31
     const int64_t off = n0 * a.s0 + n1 * a.L;
     const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
     int *spinv = (int *)(buf + a.CL * ls);

@@ -656,10 +656,10 @@ __device__ inline void spread_body(const SpreadArgs &a, const int4 item, const i
     }
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     {
-        int64_t r = item.x;
+        uint32_t r = (uint32_t)item.x; // tile index < 2^31
         for (int ax = D - 1; ax >= 0; --ax) {
-            const int qq = (int)(r % a.ntiles[ax]);
-            r /= a.ntiles[ax];
+            const int qq = (int)(r % (uint32_t)a.ntiles[ax]);
+            r /= (uint32_t)a.ntiles[ax];
             org[ax] = qq * a.tile[ax];
             teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
@@ -1042,10 +1042,10 @@ __device__ inline void spread_reduce_body(const SpreadArgs &a, const int4 it)
     // it = {tile, slot0, nslots}
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     {
-        int64_t r = it.x;
+        uint32_t r = (uint32_t)it.x; // tile index < 2^31
         for (int ax = D - 1; ax >= 0; --ax) {
-            const int q = (int)(r % a.ntiles[ax]);
-            r /= a.ntiles[ax];
+            const int q = (int)(r % (uint32_t)a.ntiles[ax]);
+            r /= (uint32_t)a.ntiles[ax];
             org[ax] = q * a.tile[ax];
             teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
@@ -1131,10 +1131,10 @@ __device__ inline void gather_body(const GatherArgs &a, const int4 item, const i
 
     int org[3] = {0, 0, 0};
     {
-        int64_t r = item.x;
+        uint32_t r = (uint32_t)item.x; // tile index < 2^31
         for (int ax = D - 1; ax >= 0; --ax) {
-            org[ax] = (int)(r % a.ntiles[ax]) * a.tile[ax];
-            r /= a.ntiles[ax];
+            org[ax] = (int)(r % (uint32_t)a.ntiles[ax]) * a.tile[ax];
+            r /= (uint32_t)a.ntiles[ax];
         }
     }
     const int R0 = a.region[0], R1 = a.region[1], R2 = (D == 3) ? a.region[2] : 1;
@@ -1249,10 +1249,10 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
     const int4 item = a.items[blockIdx.x];
     int org[3] = {0, 0, 0};
     {
-        int64_t r = item.x;
+        uint32_t r = (uint32_t)item.x; // tile index < 2^31
         for (int ax = D - 1; ax >= 0; --ax) {
-            org[ax] = (int)(r % a.ntiles[ax]) * a.tile[ax];
-            r /= a.ntiles[ax];
+            org[ax] = (int)(r % (uint32_t)a.ntiles[ax]) * a.tile[ax];
+            r /= (uint32_t)a.ntiles[ax];
         }
     }
     const int R0 = a.region[0], R1 = a.region[1], R2 = (D == 3) ? a.region[2] : 1;

@@ -212,42 +212,64 @@ __device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t s
     __syncthreads();
     const int half_in = a.rows_in / 2;
     const int cbs = __builtin_ctz(CB); // CB is a power of two
-    const int total_in = a.rows_in * CB;
+    // two columns per lane (16-byte accesses) when every row segment of the block is 16-byte aligned
+    const bool v2 = (((a.ncols | a.stride_outer | a.stride_inner | ncb) & 1) == 0) &&
+                    ((reinterpret_cast<uintptr_t>(a.data) & 15) == 0);
+    const int ws = v2 ? 1 : 0; // log2 of the columns per lane
+    const int cbw = CB >> ws, cbws = cbs - ws;
+    const int total_in = a.rows_in * cbw;
     // kLoadUnroll independent global loads in flight per thread before the LDS stores
     for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
-        float2 v[kLoadUnroll];
+        float4 v[kLoadUnroll];
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;
-            const int n = e >> cbs, j = e & (CB - 1);
-            if (e < total_in && j < ncb) v[u] = base[(int64_t)n * a.ncols + j];
+            const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
+            if (e < total_in && j < ncb) {
+                const float2 *src = base + (int64_t)n * a.ncols + j;
+                if (v2) {
+                    v[u] = *reinterpret_cast<const float4 *>(src);
+                } else {
+                    const float2 t = *src;
+                    v[u] = make_float4(t.x, t.y, 0.f, 0.f);
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;
-            const int n = e >> cbs, j = e & (CB - 1);
+            const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
             if (e < total_in && j < ncb) {
                 int p = n;
                 if (a.in_map) {
                     p = n - half_in;
                     if (p < 0) p += a.L;
                 }
-                buf[spinv[p] * estride + j] = v[u];
+                float2 *dst = buf + spinv[p] * estride + j;
+                dst[0] = make_float2(v[u].x, v[u].y);
+                if (v2) dst[1] = make_float2(v[u].z, v[u].w);
             }
         }
     }
     __syncthreads();
     fft_lines(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
     const int half_out = a.rows_out / 2;
-    for (int e = threadIdx.x; e < a.rows_out * CB; e += blockDim.x) {
-        const int n = e >> cbs, j = e & (CB - 1);
+    for (int e = threadIdx.x; e < a.rows_out * cbw; e += blockDim.x) {
+        const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
         if (j >= ncb) continue;
         int p = n;
         if (a.out_map) {
             p = n - half_out;
             if (p < 0) p += a.L;
         }
-        base[(int64_t)n * a.ncols + j] = buf[p * estride + j];
+        const float2 *src = buf + p * estride + j;
+        float2 *dst = base + (int64_t)n * a.ncols + j;
+        if (v2) {
+            const float2 lo = src[0], hi = src[1];
+            *reinterpret_cast<float4 *>(dst) = make_float4(lo.x, lo.y, hi.x, hi.y);
+        } else {
+            *dst = src[0];
+        }
     }
 }
 

@@ -98,14 +98,14 @@ static int setup_tiles(gn_plan_s *p)
         g.Kg *= g.ngtiles[a];
     }
     // LDS limits: the spread keeps per-coil planes of up to kCoilGroup coils of a tile,
-    // the gather stages tile + halo of one coil
+    // the gather stages tile + halo of a coil pair (the trajectory gradient one coil)
     const int64_t tc = (int64_t)g.tile[0] * g.tile[1] * g.tile[2];
     const int64_t ncg = std::min<int64_t>(p->nc, kCoilGroup);
     int64_t rc = 1;
     for (int a = 0; a < g.dim; ++a) rc *= std::min<int64_t>(g.gtile[a] + 2 * g.ns - 1, g.nf[a]);
     const int64_t stage = 64 * (int64_t)(g.tile[0] + g.tile[1] + g.tile[2]) * 4 + 64 * ncg * 8 + 64 * 48;
     (void)tc;
-    if (stage > 160 * 1024 || rc * 8 > 160 * 1024)
+    if (stage > 160 * 1024 || rc * kGatherCellBytes > 160 * 1024)
         return fail(GN_EINVAL, "tile shape exceeds the 160 KiB LDS budget");
 
     if (g.K > (int64_t)INT32_MAX / 2 || g.Kg > (int64_t)INT32_MAX / 2) return fail(GN_EINVAL, "too many tiles");

@@ -181,12 +181,12 @@ int build_lists(gn_batch_s *b)
             wrap |= ga.region[a] >= ga.nf[a];
             cells *= (size_t)ga.region[a];
         }
-        auto gk = std::make_tuple((int)wrap, cells * sizeof(float2));
+        auto gk = std::make_tuple((int)wrap, cells * kGatherCellBytes);
         if (!gkey.count(gk)) {
             gkey[gk] = b->gather_lists.size();
             LaunchList l;
             l.wrap = wrap;
-            l.lds = cells * sizeof(float2);
+            l.lds = cells * kGatherCellBytes;
             b->gather_lists.push_back(l);
         }
         LaunchList &gl = b->gather_lists[gkey[gk]];

@@ -292,6 +292,22 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols_batch(const ColArgs
 // ---------------------------------------------------------------------------
 
 
+__device__ inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+// Image elements e = (n, c) of a block: coil-last runs over c fastest within a
+// point n, coil-major over n fastest within a coil.  The crop pass moves pairs
+// (e, e + 1) as one 16-byte access when the run is even and 16-byte aligned
+// (img_pairs).  The pad pass keeps 8-byte accesses: pairs measured slower there.
+__device__ inline bool img_pairs(const RowArgs &a, const void *img, int64_t c0, int ncl)
+{
+    return aligned16(img) && (a.layout == GN_COIL_LAST ? ((a.nc | c0 | ncl) & 1) == 0 : (a.m & 1) == 0);
+}
+// Grid lines are contiguous runs of L per coil; pairs when L, the line offsets and G are even.
+__device__ inline bool grid_pairs(const RowArgs &a)
+{
+    return aligned16(a.grid) && ((a.L | a.s0 | a.G) & 1) == 0;
+}
+
 __device__ inline void fft_rows_pad_body(const RowArgs &a, int64_t line, int64_t chunk)
 {
     extern __shared__ float2 lds_fft[];
@@ -365,44 +381,66 @@ __device__ inline void fft_rows_crop_body(const RowArgs &a, int64_t line, int64_
     load_table(tw, a.tw, a.L, a.sign);
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
     __syncthreads();
-    const int total = ncl * a.L;
-    for (int e0 = threadIdx.x; e0 < total; e0 += kLoadUnroll * blockDim.x) {
-        float2 v[kLoadUnroll];
+    const int gw = grid_pairs(a) ? 1 : 0; // log2 of the elements per lane
+    const int ngp = (ncl * a.L) >> gw;
+    for (int q0 = threadIdx.x; q0 < ngp; q0 += kLoadUnroll * blockDim.x) {
+        float4 v[kLoadUnroll];
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
-            const int e = e0 + u * blockDim.x;
+            const int q = q0 + u * blockDim.x, e = q << gw;
             const int c = fdiv(e, a.L, rL), k = e - c * a.L;
-            if (e < total) v[u] = a.grid[(c0 + c) * a.G + off + k];
+            if (q < ngp) {
+                const float2 *src = a.grid + (c0 + c) * a.G + off + k;
+                if (gw) {
+                    v[u] = *reinterpret_cast<const float4 *>(src);
+                } else {
+                    const float2 t = *src;
+                    v[u] = make_float4(t.x, t.y, 0.f, 0.f);
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
-            const int e = e0 + u * blockDim.x;
+            const int q = q0 + u * blockDim.x, e = q << gw;
             const int c = fdiv(e, a.L, rL), k = e - c * a.L;
-            if (e < total) buf[c * ls + spinv[k]] = v[u];
+            if (q < ngp) {
+                buf[c * ls + spinv[k]] = make_float2(v[u].x, v[u].y);
+                if (gw) buf[c * ls + spinv[k + 1]] = make_float2(v[u].z, v[u].w);
+            }
         }
     }
     __syncthreads();
     fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
     const float d01 = a.scale * a.deap_a[n0] * (a.deap_b ? a.deap_b[n1] : 1.0f);
     const int half = a.m / 2;
-    for (int e = threadIdx.x; e < a.m * ncl; e += blockDim.x) {
+    const bool cl = a.layout == GN_COIL_LAST;
+    const int iw = img_pairs(a, a.img_out, c0, ncl) ? 1 : 0;
+    auto get = [&](int n, int c) {
+        int p = n - half;
+        if (p < 0) p += a.L;
+        const float d = d01 * a.deap_last[n];
+        const float2 v = buf[c * ls + p];
+        return make_float2(v.x * d, v.y * d);
+    };
+    for (int q = threadIdx.x; q < (a.m * ncl) >> iw; q += blockDim.x) {
+        const int e = q << iw;
         int n, c;
-        if (a.layout == GN_COIL_LAST) {
+        if (cl) {
             n = fdiv(e, ncl, rncl);
             c = e - n * ncl;
         } else {
             c = fdiv(e, a.m, rm);
             n = e - c * a.m;
         }
-        int p = n - half;
-        if (p < 0) p += a.L;
-        const float d = d01 * a.deap_last[n];
-        const float2 v = buf[c * ls + p];
-        const float2 o = make_float2(v.x * d, v.y * d);
-        if (a.layout == GN_COIL_LAST)
-            a.img_out[(line * a.m + n) * a.nc + c0 + c] = o;
-        else
-            a.img_out[(c0 + c) * a.nlines * a.m + line * a.m + n] = o;
+        float2 *dst = cl ? a.img_out + (line * a.m + n) * a.nc + c0 + c
+                         : a.img_out + (c0 + c) * a.nlines * a.m + line * a.m + n;
+        const float2 lo = get(n, c);
+        if (iw) {
+            const float2 hi = get(n + (cl ? 0 : 1), c + (cl ? 1 : 0));
+            *reinterpret_cast<float4 *>(dst) = make_float4(lo.x, lo.y, hi.x, hi.y);
+        } else {
+            *dst = lo;
+        }
     }
 }
 

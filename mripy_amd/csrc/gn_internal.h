@@ -21,12 +21,10 @@ constexpr int kMaxDim = 3;
 #endif
 constexpr int kGatherThreads = GN_GATHER_THREADS; // threads per gather workgroup
 constexpr int kCoilGroup = 8;       // coils accumulated together per pass
-#ifndef GN_GATHER_COILS
-#define GN_GATHER_COILS 1
-#endif
-// coils per gather work-group, staged one at a time (one coil per work-group lets
-// other work-groups hide a region's staging latency)
-constexpr int kGatherCoils = GN_GATHER_COILS;
+// coils per gather work-group: a coil pair, staged together as 16-byte cells so
+// that one LDS read serves both coils of a tap
+constexpr int kGatherCoils = 2;
+constexpr int kGatherCellBytes = 8 * kGatherCoils; // LDS bytes per staged gather cell
 #ifndef GN_SPREAD_CHUNK
 #define GN_SPREAD_CHUNK 512
 #endif

@@ -1109,13 +1109,63 @@ __global__ __launch_bounds__(kReduceThreads) void k_spread_reduce_batch(const Sp
 }
 
 // ---------------------------------------------------------------------------
-// Forward gather: one workgroup per work item (a gather tile and up to
-// kGatherChunk of its samples).  The tile plus its window halo is staged in LDS
-// one coil at a time with coalesced loads; every thread then interpolates its
-// samples from LDS, forming the separable 1-D weights in registers.
+// Forward gather: one workgroup per (work item, coil pair).  A work item is a
+// gather tile and up to kGatherChunk of its samples.  The tile plus its window
+// halo is staged in LDS as 16-byte cells holding both coils of the pair, with
+// coalesced loads; every thread then interpolates one sample of both coils from
+// LDS, one 16-byte LDS read per tap, forming the separable 1-D weights in
+// registers.  With an odd coil count the last pair's second coil is zero and
+// not written.
 // ---------------------------------------------------------------------------
 
+static_assert(kGatherCoils == 2, "the gather stages coil pairs");
 
+template <int D>
+__device__ inline void stage_region_pair(float4 *lds, const float2 *__restrict__ g0, const float2 *__restrict__ g1,
+                                         bool two, const int64_t *nf, const int *org, int R1, int R2, int RC)
+{
+    const int R12 = R1 * R2;
+    const float rR12 = 1.0f / (float)R12, rR2 = 1.0f / (float)R2;
+    for (int base = threadIdx.x; base < RC; base += kStageUnroll * blockDim.x) {
+        float4 v[kStageUnroll];
+#pragma unroll
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int rr = base + u * blockDim.x;
+            if (rr < RC) {
+                const int r0 = fdiv_small(rr, R12, rR12);
+                const int rem = rr - r0 * R12;
+                const int r1 = D == 3 ? fdiv_small(rem, R2, rR2) : rem;
+                const int r2 = D == 3 ? rem - r1 * R2 : 0;
+                int q0 = org[0] + r0;
+                if (q0 >= nf[0]) q0 -= (int)nf[0];
+                int q1 = org[1] + r1;
+                if (q1 >= nf[1]) q1 -= (int)nf[1];
+                int64_t gi = (int64_t)q0 * nf[1] + q1;
+                if (D == 3) {
+                    int q2 = org[2] + r2;
+                    if (q2 >= nf[2]) q2 -= (int)nf[2];
+                    gi = gi * nf[2] + q2;
+                }
+                const float2 x0 = g0[gi], x1 = two ? g1[gi] : make_float2(0.f, 0.f);
+                v[u] = make_float4(x0.x, x0.y, x1.x, x1.y);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int rr = base + u * blockDim.x;
+            if (rr < RC) lds[rr] = v[u];
+        }
+    }
+}
+
+// acc += w * v on both coils of a staged cell
+__device__ inline void fma4(float4 &acc, float w, const float4 v)
+{
+    acc.x += w * v.x;
+    acc.y += w * v.y;
+    acc.z += w * v.z;
+    acc.w += w * v.w;
+}
 
 // Tap weights use the hardware exp2 (v_exp_f32) directly.  The exponent is
 // coef * u^2 with coef = pi (rat - 1/2) / (rat ns ln 2) and |u| <= ns + 1, so every
@@ -1125,9 +1175,9 @@ template <int D, int NS, bool WRAP>
 __device__ inline void gather_body(const GatherArgs &a, const int4 item, const int cgi)
 {
     constexpr int W = 2 * NS;
-    extern __shared__ float2 lds_gather[];
+    extern __shared__ float4 lds_pair[];
     const int cg0 = cgi * kGatherCoils;
-    const int ncg = (int)min((int64_t)kGatherCoils, a.nc - cg0);
+    const bool two = cg0 + 1 < a.nc;
 
     int org[3] = {0, 0, 0};
     {
@@ -1139,74 +1189,68 @@ __device__ inline void gather_body(const GatherArgs &a, const int4 item, const i
     }
     const int R0 = a.region[0], R1 = a.region[1], R2 = (D == 3) ? a.region[2] : 1;
     const int RC = R0 * R1 * R2;
-    const int R12 = R1 * R2;
 
-    for (int cc = 0; cc < ncg; ++cc) {
-        const float2 *g = a.grid + (int64_t)(cg0 + cc) * a.G;
-        __syncthreads();
-        stage_region<D>(lds_gather, g, a.nf, org, R1, R2, RC);
-        __syncthreads();
-        for (int j = item.y + threadIdx.x; j < item.z; j += kGatherThreads) {
-            const int4 rc = a.rec_cell[j];
-            const float4 rf = a.rec_frac[j];
-            int l0 = rc.x - org[0];
-            if (l0 < 0) l0 += (int)a.nf[0];
-            int l1 = rc.y - org[1];
-            if (l1 < 0) l1 += (int)a.nf[1];
-            int l2 = 0;
+    const float2 *g0 = a.grid + (int64_t)cg0 * a.G;
+    stage_region_pair<D>(lds_pair, g0, g0 + a.G, two, a.nf, org, R1, R2, RC);
+    __syncthreads();
+    for (int j = item.y + threadIdx.x; j < item.z; j += kGatherThreads) {
+        const int4 rc = a.rec_cell[j];
+        const float4 rf = a.rec_frac[j];
+        int l0 = rc.x - org[0];
+        if (l0 < 0) l0 += (int)a.nf[0];
+        int l1 = rc.y - org[1];
+        if (l1 < 0) l1 += (int)a.nf[1];
+        int l2 = 0;
+        if (D == 3) {
+            l2 = rc.z - org[2];
+            if (l2 < 0) l2 += (int)a.nf[2];
+        }
+        float w1[W], w2[D == 3 ? W : 1];
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const float u1 = rf.y + (float)(NS - t);
+            w1[t] = __builtin_amdgcn_exp2f(-a.coef[1] * u1 * u1);
             if (D == 3) {
-                l2 = rc.z - org[2];
-                if (l2 < 0) l2 += (int)a.nf[2];
+                const float u2 = rf.z + (float)(NS - t);
+                w2[t] = __builtin_amdgcn_exp2f(-a.coef[2] * u2 * u2);
             }
-            float w1[W], w2[D == 3 ? W : 1];
-#pragma unroll
-            for (int t = 0; t < W; ++t) {
-                const float u1 = rf.y + (float)(NS - t);
-                w1[t] = __builtin_amdgcn_exp2f(-a.coef[1] * u1 * u1);
-                if (D == 3) {
-                    const float u2 = rf.z + (float)(NS - t);
-                    w2[t] = __builtin_amdgcn_exp2f(-a.coef[2] * u2 * u2);
-                }
-            }
-            float ar = 0.f, ai = 0.f;
+        }
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll 1
-            for (int t0 = 0; t0 < W; ++t0) {
-                const float u0 = rf.x + (float)(NS - t0);
-                const float w0 = __builtin_amdgcn_exp2f(-a.coef[0] * u0 * u0);
-                int r0 = l0 + t0;
-                if (WRAP && r0 >= R0) r0 -= (int)a.nf[0]; // only when the region spans the whole axis
-                float pr = 0.f, pi = 0.f;
+        for (int t0 = 0; t0 < W; ++t0) {
+            const float u0 = rf.x + (float)(NS - t0);
+            const float w0 = __builtin_amdgcn_exp2f(-a.coef[0] * u0 * u0);
+            int r0 = l0 + t0;
+            if (WRAP && r0 >= R0) r0 -= (int)a.nf[0]; // only when the region spans the whole axis
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-                for (int t1 = 0; t1 < W; ++t1) {
-                    int r1 = l1 + t1;
-                    if (WRAP && r1 >= R1) r1 -= (int)a.nf[1];
-                    if (D == 3) {
-                        const int rowb = (r0 * R1 + r1) * R2;
-                        float qr = 0.f, qi = 0.f;
+            for (int t1 = 0; t1 < W; ++t1) {
+                int r1 = l1 + t1;
+                if (WRAP && r1 >= R1) r1 -= (int)a.nf[1];
+                if (D == 3) {
+                    const int rowb = (r0 * R1 + r1) * R2;
+                    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-                        for (int t2 = 0; t2 < W; ++t2) {
-                            int r2 = l2 + t2;
-                            if (WRAP && r2 >= R2) r2 -= (int)a.nf[2];
-                            const float2 gv = lds_gather[rowb + r2];
-                            qr += w2[t2] * gv.x;
-                            qi += w2[t2] * gv.y;
-                        }
-                        pr += w1[t1] * qr;
-                        pi += w1[t1] * qi;
-                    } else {
-                        const float2 gv = lds_gather[r0 * R1 + r1];
-                        pr += w1[t1] * gv.x;
-                        pi += w1[t1] * gv.y;
+                    for (int t2 = 0; t2 < W; ++t2) {
+                        int r2 = l2 + t2;
+                        if (WRAP && r2 >= R2) r2 -= (int)a.nf[2];
+                        fma4(q, w2[t2], lds_pair[rowb + r2]);
                     }
+                    fma4(p, w1[t1], q);
+                } else {
+                    fma4(p, w1[t1], lds_pair[r0 * R1 + r1]);
                 }
-                ar += w0 * pr;
-                ai += w0 * pi;
             }
-            const int64_t coil = cg0 + cc;
-            if (a.layout == GN_COIL_LAST)
-                a.out[(int64_t)rc.w * a.nc + coil] = make_float2(ar, ai);
-            else
-                a.out[coil * a.M + rc.w] = make_float2(ar, ai);
+            fma4(acc, w0, p);
+        }
+        const float2 o0 = make_float2(acc.x, acc.y), o1 = make_float2(acc.z, acc.w);
+        if (a.layout == GN_COIL_LAST) {
+            float2 *o = a.out + (int64_t)rc.w * a.nc + cg0;
+            o[0] = o0;
+            if (two) o[1] = o1;
+        } else {
+            a.out[(int64_t)cg0 * a.M + rc.w] = o0;
+            if (two) a.out[(int64_t)(cg0 + 1) * a.M + rc.w] = o1;
         }
     }
 }
@@ -1561,9 +1605,9 @@ void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
         tile[0] = 4;
         tile[1] = 4;
         tile[2] = 8;
-        gtile[0] = 8;
-        gtile[1] = 16;
-        gtile[2] = 16;
+        gtile[0] = 12;
+        gtile[1] = 8;
+        gtile[2] = 8;
     } else {
         tile[0] = 8;
         tile[1] = 16;
@@ -1815,7 +1859,7 @@ hipError_t spread_dispatch(const SpreadArgs &a, int nitems, int nritems, const i
 template <int D, int NS>
 hipError_t gather_dispatch(const GatherArgs &a, int nitems, int64_t nc, hipStream_t st)
 {
-    const size_t lds = (size_t)a.region[0] * a.region[1] * (D == 3 ? a.region[2] : 1) * sizeof(float2);
+    const size_t lds = (size_t)a.region[0] * a.region[1] * (D == 3 ? a.region[2] : 1) * kGatherCellBytes;
     bool wrap = false;
     for (int ax = 0; ax < D; ++ax) wrap |= a.region[ax] >= a.nf[ax];
     dim3 grid((unsigned)nitems, (unsigned)((nc + kGatherCoils - 1) / kGatherCoils));

@@ -571,6 +571,7 @@ int gn_plan_destroy(gn_plan *p)
     hipFree(p->rec.reduce_items);
     hipFree(p->rec.gather_items);
     hipFree(p->rec.partial);
+    hipFree(p->rec.occ_mask);
     hipFree(p->grid);
     hipFree(p->ybuf);
     hipFree(p->deap);

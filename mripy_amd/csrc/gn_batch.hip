@@ -554,12 +554,12 @@ std::vector<std::vector<int>> candidates(int dim)
     std::vector<std::vector<int>> out;
     if (dim == 2) {
         const int sp[][2] = {{8, 16}, {16, 8}, {16, 16}, {8, 32}};
-        const int ga[][2] = {{32, 32}, {16, 32}, {32, 64}, {64, 64}};
+        const int ga[][2] = {{24, 24}, {16, 32}, {32, 32}, {32, 64}};
         for (auto &s : sp)
             for (auto &g : ga) out.push_back({s[0], s[1], 1, g[0], g[1], 1});
     } else {
         const int sp[][3] = {{4, 4, 8}, {4, 8, 8}, {8, 8, 8}, {4, 4, 16}};
-        const int ga[][3] = {{8, 8, 16}, {8, 16, 16}, {16, 16, 16}, {8, 8, 32}};
+        const int ga[][3] = {{10, 8, 12}, {10, 8, 8}, {8, 8, 16}, {8, 16, 16}};
         for (auto &s : sp)
             for (auto &g : ga) out.push_back({s[0], s[1], s[2], g[0], g[1], g[2]});
     }

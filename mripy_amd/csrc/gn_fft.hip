@@ -191,6 +191,15 @@ __device__ inline void load_table(float2 *dst, const float2 *src, int n, float s
 // ---------------------------------------------------------------------------
 
 
+// Row mask word of grid column c (c < 2^31) for the masked leading-axis passes.
+__device__ inline unsigned long long col_mask(const ColArgs &a, const unsigned long long *mask, int64_t c)
+{
+    const uint32_t y = (uint32_t)c / (uint32_t)a.mcz, z = (uint32_t)c - y * (uint32_t)a.mcz;
+    return mask[(y / (uint32_t)a.mt1) * (uint32_t)a.mnt2 + z / (uint32_t)a.mt2];
+}
+
+// MASK: 0 none, 1 read mask (cells outside it enter the FFT as zeros)
+template <int MASK>
 __device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t slab)
 {
     extern __shared__ float2 lds_fft[];
@@ -204,12 +213,6 @@ __device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t s
     // slab and slab_inner are below 2^31: 32-bit division
     const uint32_t so = (uint32_t)slab / (uint32_t)a.slab_inner, si = (uint32_t)slab - so * (uint32_t)a.slab_inner;
     float2 *base = a.data + so * a.stride_outer + si * a.stride_inner + col0;
-    int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
-    load_table(tw, a.tw, a.L, a.sign);
-    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
-    if (a.rows_in < a.L) // pruned input: zero every slot first
-        for (int e = threadIdx.x; e < a.L * estride; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
-    __syncthreads();
     const int half_in = a.rows_in / 2;
     const int cbs = __builtin_ctz(CB); // CB is a power of two
     // two columns per lane (16-byte accesses) when every row segment of the block is 16-byte aligned
@@ -218,6 +221,22 @@ __device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t s
     const int ws = v2 ? 1 : 0; // log2 of the columns per lane
     const int cbw = CB >> ws, cbws = cbs - ws;
     const int total_in = a.rows_in * cbw;
+    // Row masks of this lane's columns: the lane index fixes the column (blockDim.x is a
+    // multiple of the columns per row), so one word per column serves every row.  They
+    // are fetched before the table staging so that both loads overlap.
+    const float rmrow = 1.0f / (float)a.mrow;
+    unsigned long long mk0 = ~0ull, mk1 = ~0ull;
+    if (MASK) {
+        const int j = (threadIdx.x & (cbw - 1)) << ws;
+        if (j < ncb) mk0 = col_mask(a, a.rmask, col0 + j);
+        mk1 = (v2 && j + 1 < ncb) ? col_mask(a, a.rmask, col0 + j + 1) : mk0;
+    }
+    int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
+    load_table(tw, a.tw, a.L, a.sign);
+    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
+    if (a.rows_in < a.L) // pruned input: zero every slot first
+        for (int e = threadIdx.x; e < a.L * estride; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
+    __syncthreads();
     // kLoadUnroll independent global loads in flight per thread before the LDS stores
     for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
         float4 v[kLoadUnroll];
@@ -226,12 +245,23 @@ __device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t s
             const int e = e0 + u * blockDim.x;
             const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
             if (e < total_in && j < ncb) {
-                const float2 *src = base + (int64_t)n * a.ncols + j;
-                if (v2) {
-                    v[u] = *reinterpret_cast<const float4 *>(src);
-                } else {
-                    const float2 t = *src;
-                    v[u] = make_float4(t.x, t.y, 0.f, 0.f);
+                bool on0 = true, on1 = true;
+                if (MASK == 1) { // cells of tiles the spread did not write enter as zeros
+                    const int b = fdiv(n, a.mrow, rmrow);
+                    on0 = (mk0 >> b) & 1ull;
+                    on1 = (mk1 >> b) & 1ull;
+                }
+                v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (on0 || on1) {
+                    const float2 *src = base + (int64_t)n * a.ncols + j;
+                    if (v2) {
+                        v[u] = *reinterpret_cast<const float4 *>(src);
+                        if (!on0) v[u].x = v[u].y = 0.f;
+                        if (!on1) v[u].z = v[u].w = 0.f;
+                    } else {
+                        const float2 t = *src;
+                        v[u] = make_float4(t.x, t.y, 0.f, 0.f);
+                    }
                 }
             }
         }
@@ -273,7 +303,11 @@ __device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t s
     }
 }
 
-__global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a) { fft_cols_body(a, blockIdx.x, blockIdx.y); }
+template <int MASK>
+__global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
+{
+    fft_cols_body<MASK>(a, blockIdx.x, blockIdx.y);
+}
 
 // Batched column passes: items[b] = {argument index, column block, slab, 0}.
 __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols_batch(const ColArgs *__restrict__ args,
@@ -281,7 +315,7 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols_batch(const ColArgs
 {
     const int4 it = items[blockIdx.x];
     const ColArgs &a = args[it.x]; // read in place: uniform scalar loads
-    fft_cols_body(a, it.y, it.z);
+    fft_cols_body<0>(a, it.y, it.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -552,8 +586,13 @@ hipError_t launch_cols(const ColArgs &a, hipStream_t st)
 {
     dim3 grid((unsigned)((a.ncols + a.CB - 1) / a.CB), (unsigned)a.nslabs);
     const size_t lds = col_lds(a.L, a.CB);
-    allow_lds((const void *)k_fft_cols, lds);
-    hipLaunchKernelGGL(k_fft_cols, grid, dim3(kFFTColThreads), lds, st, a);
+    if (a.rmask) {
+        allow_lds((const void *)k_fft_cols<1>, lds);
+        hipLaunchKernelGGL(k_fft_cols<1>, grid, dim3(kFFTColThreads), lds, st, a);
+    } else {
+        allow_lds((const void *)k_fft_cols<0>, lds);
+        hipLaunchKernelGGL(k_fft_cols<0>, grid, dim3(kFFTColThreads), lds, st, a);
+    }
     return hipGetLastError();
 }
 
@@ -722,6 +761,18 @@ int pruned_type2_rows(gn_plan_s *p, const float2 *image, float scale, float sign
     return e == hipSuccess ? GN_OK : hip_fail(e, "FFT pass A");
 }
 
+// Mask geometry of the leading-axis pass: rows per mask bit, and the spread tiling of
+// the other axes that maps a grid column to its mask word.
+static void set_mask_geometry(const gn_plan_s *p, ColArgs &a)
+{
+    const Geometry &g = p->geo;
+    a.mrow = g.tile[0] * p->rec.mask_k;
+    a.mcz = g.dim == 3 ? (int)g.nf[2] : 1;
+    a.mt1 = g.tile[1];
+    a.mt2 = g.dim == 3 ? g.tile[2] : 1;
+    a.mnt2 = g.dim == 3 ? g.ntiles[2] : 1;
+}
+
 // Type 2, passes B (3-D) and C: the remaining axes, expanding m rows to nf.
 int pruned_type2_cols(gn_plan_s *p, float sign)
 {
@@ -735,10 +786,16 @@ int pruned_type2_cols(gn_plan_s *p, float sign)
 }
 
 // Type 1, passes C' and B' (3-D): keep m of nf rows on the leading axes (in place).
+// Pass C' reads only the spread tiles that hold pairs (occupancy mask); the spread
+// does not write the others.
 int pruned_type1_cols(gn_plan_s *p, float sign)
 {
     ColArgs a[2];
     const int n = pruned_col_args(p, false, sign, a);
+    if (p->rec.occ_mask) {
+        a[0].rmask = p->rec.occ_mask;
+        set_mask_geometry(p, a[0]);
+    }
     for (int k = 0; k < n; ++k) {
         hipError_t e = launch_cols(a[k], p->stream);
         if (e != hipSuccess) return hip_fail(e, k == 0 ? "FFT pass C'" : "FFT pass B'");

@@ -62,6 +62,12 @@ struct ColArgs {
     Stages st;
     const float2 *tw;
     const int *pinv; // FFT position -> LDS slot (digit reversal)
+    // Optional row mask over the grid cells of this pass (the first adjoint pass of a
+    // plain plan): element (row n, column c) is read only if bit n / mrow of
+    // rmask[(y / mt1) * mnt2 + z / mt2] is set, where y = c / mcz, z = c % mcz.
+    // Unread elements enter the FFT as zeros.  Null: every element is read.
+    const unsigned long long *rmask = nullptr;
+    int mrow = 1, mcz = 1, mt1 = 1, mt2 = 1, mnt2 = 1;
 };
 
 struct RowArgs {
@@ -112,6 +118,11 @@ struct Records {
     int n_gather_items = 0;
     float2 *partial = nullptr;    // n_slots * nc * TC partial tiles
     int64_t partial_cap = 0;
+    // Occupancy row masks per column of spread tiles (ntiles[1] * ntiles[2] words):
+    // bit b is set when one of the leading-axis tiles [b * mask_k, (b + 1) * mask_k)
+    // of that column has (tile, sample) pairs.
+    unsigned long long *occ_mask = nullptr;
+    int mask_k = 1;
 };
 
 struct Geometry {
@@ -154,6 +165,12 @@ struct SpreadArgs {
     const float2 *in;
     float2 *grid;
     float2 *partial;
+    // Occupancy masks of the spread tiles (bit t0 / occ_k of occ[t1 * ntiles[2] + t2]
+    // set when some tile of that row group has pairs).  When set, an empty tile whose
+    // group bit is clear is not written: the pass that reads the grid next skips it.
+    // Null: every empty tile is written with zeros.
+    const unsigned long long *occ = nullptr;
+    int occ_k = 1;
 };
 
 struct GatherArgs {

@@ -467,6 +467,15 @@ __global__ __launch_bounds__(256) void k_permute_pairs(int64_t n, const int *__r
 constexpr int kSpreadBatch = 64;
 constexpr int kSpreadWavesMax = 8;
 
+// Spread tile t has no pairs and no later pass reads it: its cells need not be written.
+__device__ inline bool tile_skippable(const SpreadArgs &a, const int4 item)
+{
+    if (item.y < item.z || !a.occ) return false;
+    const uint32_t nt12 = (uint32_t)a.ntiles[1] * (uint32_t)a.ntiles[2];
+    const uint32_t t0 = (uint32_t)item.x / nt12, rest = (uint32_t)item.x - t0 * nt12;
+    return ((a.occ[rest] >> (t0 / (uint32_t)a.occ_k)) & 1ull) == 0;
+}
+
 template <int D, int NCG>
 __device__ inline void spread_body(const SpreadArgs &a, const int4 item, const int cgi, const int bz)
 {
@@ -478,6 +487,7 @@ __device__ inline void spread_body(const SpreadArgs &a, const int4 item, const i
     int4 *s_pa = (int4 *)(s_msk + kSpreadBatch * 4);
     float4 *s_pb = (float4 *)(s_pa + kSpreadBatch);
 
+    if (tile_skippable(a, item)) return; // uniform over the workgroup
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
@@ -735,6 +745,7 @@ __device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, co
 {
     constexpr int B = kSpreadBatch;
     extern __shared__ float lds_sp[];
+    if (tile_skippable(a, item)) return; // uniform over the workgroup
     // LDS: weight tables [B + 1][ttp] | reals [B + 1][16] | masks [B][4] | per-axis window
     //      parameters [B][4] {l, c, ns - j, t} | per-wave pair lists [waves][4][B];
     //      row B of the tables and reals is the all-zero pair
@@ -1605,15 +1616,15 @@ void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
         tile[0] = 4;
         tile[1] = 4;
         tile[2] = 8;
-        gtile[0] = 12;
+        gtile[0] = 10;
         gtile[1] = 8;
-        gtile[2] = 8;
+        gtile[2] = 12;
     } else {
         tile[0] = 8;
         tile[1] = 16;
         tile[2] = 1;
-        gtile[0] = 32;
-        gtile[1] = 32;
+        gtile[0] = 24;
+        gtile[1] = 24;
         gtile[2] = 1;
     }
     (void)ns;
@@ -1766,6 +1777,21 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
     R.n_reduce_items = (int)ritems.size();
     R.n_gather_items = (int)gitems.size();
     R.n_slots = slots;
+    // Occupancy row masks for the first adjoint FFT pass (one word per column of spread
+    // tiles): the spread leaves tiles without pairs unwritten where the pass skips them.
+    {
+        const int64_t ncolt = (int64_t)g.ntiles[1] * g.ntiles[2]; // ntiles[2] = 1 in 2-D
+        const int kk = (g.ntiles[0] + 63) / 64;                   // leading-axis tiles per bit
+        std::vector<unsigned long long> occ((size_t)ncolt, 0ull);
+        for (const int4 &it : sitems)
+            if (it.z > it.y) {
+                const int64_t t0 = it.x / ncolt;
+                occ[(size_t)(it.x - t0 * ncolt)] |= 1ull << (t0 / kk);
+            }
+        if ((rc = dev_alloc(&R.occ_mask, (size_t)ncolt, "tile masks"))) return rc;
+        GN_HIP(hipMemcpy(R.occ_mask, occ.data(), sizeof(unsigned long long) * ncolt, hipMemcpyHostToDevice));
+        R.mask_k = kk;
+    }
     const int64_t TC = (int64_t)g.tile[0] * g.tile[1] * g.tile[2];
     const int64_t need = (int64_t)slots * p->nc * TC;
     if (need > R.partial_cap) {
@@ -1977,7 +2003,12 @@ SpreadArgs make_spread_args(const gn_plan_s *p, const float2 *samples, int use_d
 int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
 {
     const Geometry &g = p->geo;
-    const SpreadArgs a = make_spread_args(p, samples, use_dcf);
+    SpreadArgs a = make_spread_args(p, samples, use_dcf);
+    if (p->fft_custom && p->fft_engine == GN_FFT_AUTO) {
+        // the pruned pass C' reads only occupied tiles: empty ones need no zeros
+        a.occ = p->rec.occ_mask;
+        a.occ_k = p->rec.mask_k;
+    }
     hipError_t e = spread_any(g.dim, g.ns, a, p->rec.n_spread_items, p->rec.n_reduce_items, p->rec.reduce_items, p->nc,
                               p->stream);
     if (e != hipSuccess) return hip_fail(e, "spread launch");

@@ -169,3 +169,33 @@ def test_tiles_larger_than_one_workgroup(dim):
     P.set_points(coords)
     c = (rng.standard_normal((M, nc)) + 1j * rng.standard_normal((M, nc))).astype(np.complex64)
     assert rel(P.type1(c), C.type1(coords, c, p)) <= TOL
+
+
+@pytest.mark.parametrize("dim,modes,nc", [(3, (32, 32, 32), 3), (2, (64, 48), 2), (2, (600, 120), 2)])
+def test_unwritten_grid_cells_stay_out_of_results(dim, modes, nc):
+    """Samples inside a small ball leave most spread tiles empty.  With the pruned FFT
+    passes the spread leaves empty tiles unwritten and the first adjoint pass reads
+    zeros for them.  Alternating type 2, type 1 and the rocFFT engine on one plan
+    leaves stale values of other transforms in those cells, and none may reach a
+    result.  (600, 120)
+    has 150 spread tiles along the leading axis, so one mask bit covers three tiles
+    and the spread writes zeros for the empty tiles of occupied groups."""
+    print("case: unwritten grid cells", dim, modes, nc, flush=True)
+    rng = np.random.default_rng(40 + len(modes) + modes[0])
+    M = 6000
+    u = rng.standard_normal((M, dim))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = 0.35 * np.pi * rng.random(M) ** (1.0 / dim)
+    coords = [(u[:, a] * r).astype(np.float32) for a in range(dim)]
+    p = O.Params(modes, rat=2, nspread=3)
+    P = _plan(modes, nc, rat=2, nspread=3)
+    P.set_points(coords)
+    img = (rng.standard_normal(modes + (nc,)) + 1j * rng.standard_normal(modes + (nc,))).astype(np.complex64)
+    c = (rng.standard_normal((M, nc)) + 1j * rng.standard_normal((M, nc))).astype(np.complex64)
+    ref2 = C.type2(coords, img, p)
+    ref1 = C.type1(coords, c, p)
+    for engine in ("auto", "auto", "rocfft", "auto"):
+        P.set_fft_engine(engine)
+        e2 = rel(P.type2(img), ref2)
+        e1 = rel(P.type1(c), ref1)
+        assert e2 <= TOL and e1 <= TOL, (engine, e2, e1)

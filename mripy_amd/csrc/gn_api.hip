@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -539,6 +540,12 @@ int gn_plan_sample_order(const gn_plan *p, int *order)
     hipError_t e = hipMemcpy2D(order, sizeof(int), (const char *)p->rec.cell + 3 * sizeof(int), sizeof(int4),
                                sizeof(int), (size_t)p->M, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "sample order copy");
+    // The records of a gather tile are stored in LDS-bank order; the counting sort is
+    // stable, so its order within a tile is ascending original index.
+    std::vector<int> start((size_t)p->geo.Kg + 1);
+    e = hipMemcpy(start.data(), p->rec.bin_start, sizeof(int) * start.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "sample order copy");
+    for (int64_t k = 0; k < p->geo.Kg; ++k) std::sort(order + start[(size_t)k], order + start[(size_t)k + 1]);
     return GN_OK;
 }
 

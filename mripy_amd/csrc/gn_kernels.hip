@@ -1631,6 +1631,80 @@ void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
     (void)nf;
 }
 
+// Orders the records of one gather work item for conflict-free LDS reads in the
+// gather.  Every tap adds the same offset to all lanes' staged-cell addresses, so if
+// the lanes of each group of 16 start at cells of distinct residues mod 16 (the
+// 16-byte cells map to 16 disjoint groups of four LDS banks), every tap of that group
+// is free of bank conflicts.  Samples are placed by (rank within their residue class,
+// residue): position = sum_v min(size_v, k) + #{v < r : size_v > k} for rank k and
+// residue r.  Ranks are taken in record order with wave ballots, so the permutation is
+// deterministic.  Each sample's result does not depend on its lane.
+static_assert(kGatherChunk <= kGatherThreads, "one record per thread in the bank ordering");
+
+template <int D>
+__global__ __launch_bounds__(kGatherThreads) void k_bank_order(GatherArgs a, int4 *cell, float4 *frac)
+{
+    constexpr int kWaves = kGatherThreads / 64;
+    __shared__ int s_cnt[kWaves][16];
+    __shared__ int s_sz[16];
+    const int4 item = a.items[blockIdx.x];
+    const int n = item.z - item.y;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    int org[3] = {0, 0, 0};
+    {
+        uint32_t r = (uint32_t)item.x; // tile index < 2^31
+        for (int ax = D - 1; ax >= 0; --ax) {
+            org[ax] = (int)(r % (uint32_t)a.ntiles[ax]) * a.tile[ax];
+            r /= (uint32_t)a.ntiles[ax];
+        }
+    }
+    int4 rc = make_int4(0, 0, 0, 0);
+    float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
+    int res = 16; // no sample
+    if (tid < n) {
+        rc = cell[item.y + tid];
+        rf = frac[item.y + tid];
+        int l0 = rc.x - org[0];
+        if (l0 < 0) l0 += (int)a.nf[0];
+        int l1 = rc.y - org[1];
+        if (l1 < 0) l1 += (int)a.nf[1];
+        int addr = l0 * a.region[1] + l1;
+        if (D == 3) {
+            int l2 = rc.z - org[2];
+            if (l2 < 0) l2 += (int)a.nf[2];
+            addr = addr * a.region[2] + l2;
+        }
+        res = addr & 15;
+    }
+    int rank = 0;
+    for (int v = 0; v < 16; ++v) {
+        const unsigned long long m = __ballot(res == v);
+        if (res == v) rank = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_cnt[wave][v] = __popcll(m);
+    }
+    __syncthreads();
+    if (tid < 16) {
+        int sum = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const int c = s_cnt[w][tid];
+            s_cnt[w][tid] = sum;
+            sum += c;
+        }
+        s_sz[tid] = sum;
+    }
+    __syncthreads();
+    if (tid < n) {
+        const int k = s_cnt[wave][res] + rank;
+        int pos = 0;
+        for (int v = 0; v < 16; ++v) {
+            const int sz = s_sz[v];
+            pos += min(sz, k) + ((v < res && sz > k) ? 1 : 0);
+        }
+        cell[item.y + pos] = rc; // every record of the item was read before the barriers
+        frac[item.y + pos] = rf;
+    }
+}
+
 #define GN_HIP(call)                                                                                                   \
     do {                                                                                                               \
         hipError_t e_ = (call);                                                                                        \
@@ -1791,6 +1865,16 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         if ((rc = dev_alloc(&R.occ_mask, (size_t)ncolt, "tile masks"))) return rc;
         GN_HIP(hipMemcpy(R.occ_mask, occ.data(), sizeof(unsigned long long) * ncolt, hipMemcpyHostToDevice));
         R.mask_k = kk;
+    }
+    if (R.n_gather_items > 0) {
+        const GatherArgs ga = make_gather_args(p, nullptr);
+        if (g.dim == 3)
+            hipLaunchKernelGGL(k_bank_order<3>, dim3((unsigned)R.n_gather_items), dim3(kGatherThreads), 0, st, ga,
+                               R.cell, R.frac);
+        else
+            hipLaunchKernelGGL(k_bank_order<2>, dim3((unsigned)R.n_gather_items), dim3(kGatherThreads), 0, st, ga,
+                               R.cell, R.frac);
+        GN_HIP(hipGetLastError());
     }
     const int64_t TC = (int64_t)g.tile[0] * g.tile[1] * g.tile[2];
     const int64_t need = (int64_t)slots * p->nc * TC;

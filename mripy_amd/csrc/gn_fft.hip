@@ -200,7 +200,7 @@ __device__ inline unsigned long long col_mask(const ColArgs &a, const unsigned l
 
 // MASK: 0 none, 1 read mask (cells outside it enter the FFT as zeros)
 template <int MASK>
-__device__ inline void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t slab)
+__device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, int64_t slab)
 {
     extern __shared__ float2 lds_fft[];
     float2 *tw = lds_fft;
@@ -342,7 +342,7 @@ __device__ inline bool grid_pairs(const RowArgs &a)
     return aligned16(a.grid) && ((a.L | a.s0 | a.G) & 1) == 0;
 }
 
-__device__ inline void fft_rows_pad_body(const RowArgs &a, int64_t line, int64_t chunk)
+__device__ __forceinline__ void fft_rows_pad_body(const RowArgs &a, int64_t line, int64_t chunk)
 {
     extern __shared__ float2 lds_fft[];
     float2 *tw = lds_fft;
@@ -400,7 +400,7 @@ __device__ inline void fft_rows_pad_body(const RowArgs &a, int64_t line, int64_t
     }
 }
 
-__device__ inline void fft_rows_crop_body(const RowArgs &a, int64_t line, int64_t chunk)
+__device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t line, int64_t chunk)
 {
     extern __shared__ float2 lds_fft[];
     float2 *tw = lds_fft;

@@ -477,7 +477,7 @@ __device__ inline bool tile_skippable(const SpreadArgs &a, const int4 item)
 }
 
 template <int D, int NCG>
-__device__ inline void spread_body(const SpreadArgs &a, const int4 item, const int cgi, const int bz)
+__device__ __forceinline__ void spread_body(const SpreadArgs &a, const int4 item, const int cgi, const int bz)
 {
     extern __shared__ float lds_sp[];
     // LDS: weight tables [batch][TT] | coil values [batch][NCG] float2 | masks [batch][3]
@@ -741,7 +741,7 @@ inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
 }
 
 template <int D, int NCG>
-__device__ inline void spread_mfma_body(const SpreadArgs &a, const int4 item, const int cgi, const int bz)
+__device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4 item, const int cgi, const int bz)
 {
     constexpr int B = kSpreadBatch;
     extern __shared__ float lds_sp[];
@@ -1048,7 +1048,7 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_mfma_batch(cons
 // one thread per (coil, cell) of a tile (kReduceThreads covers 8 coils of 128 cells).
 constexpr int kReduceThreads = 1024;
 template <int D>
-__device__ inline void spread_reduce_body(const SpreadArgs &a, const int4 it)
+__device__ __forceinline__ void spread_reduce_body(const SpreadArgs &a, const int4 it)
 {
     // it = {tile, slot0, nslots}
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
@@ -1183,7 +1183,7 @@ __device__ inline void fma4(float4 &acc, float w, const float4 v)
 // weight is >= 2^-82 for nspread <= 16: far from the denormal range, where the
 // hardware exp2 and exp2f agree bit for bit.
 template <int D, int NS, bool WRAP>
-__device__ inline void gather_body(const GatherArgs &a, const int4 item, const int cgi)
+__device__ __forceinline__ void gather_body(const GatherArgs &a, const int4 item, const int cgi)
 {
     constexpr int W = 2 * NS;
     extern __shared__ float4 lds_pair[];
@@ -1409,7 +1409,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
 
 
 template <int D>
-__device__ inline void pad_body(const PrepostArgs &a, const float2 *__restrict__ img, float2 *__restrict__ grid)
+__device__ __forceinline__ void pad_body(const PrepostArgs &a, const float2 *__restrict__ img, float2 *__restrict__ grid)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t gidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < a.G; gidx += stride) {
@@ -1466,7 +1466,7 @@ __global__ __launch_bounds__(256) void k_pad_batch(const PrepostArgs *__restrict
 }
 
 template <int D>
-__device__ inline void crop_body(const PrepostArgs &a, const float2 *__restrict__ grid, float2 *__restrict__ img)
+__device__ __forceinline__ void crop_body(const PrepostArgs &a, const float2 *__restrict__ grid, float2 *__restrict__ img)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += stride) {

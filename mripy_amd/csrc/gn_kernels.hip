@@ -831,12 +831,26 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
                                           __int_as_float(pbi.w));
             const float scale = a.use_dcf ? pb.w : 1.0f;
             float2 v[NCG];
+            // coil-last with an even coil count: the pair's coils are contiguous 16-byte pairs
+            const bool v4 = NCG >= 2 && a.layout == GN_COIL_LAST && ((a.nc | cg0) & 1) == 0 &&
+                            (reinterpret_cast<uintptr_t>(a.in) & 15) == 0;
+            if (v4) {
+                const float4 *src = reinterpret_cast<const float4 *>(a.in + (int64_t)pa.x * a.nc + cg0);
 #pragma unroll
-            for (int c = 0; c < NCG; ++c) {
-                v[c] = make_float2(0.f, 0.f);
-                if (c < ncg) {
-                    const int64_t coil = cg0 + c;
-                    v[c] = (a.layout == GN_COIL_LAST) ? a.in[(int64_t)pa.x * a.nc + coil] : a.in[coil * a.M + pa.x];
+                for (int c2 = 0; c2 < (NCG + 1) / 2; ++c2) {
+                    float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (2 * c2 < ncg) t = src[c2];
+                    v[2 * c2] = make_float2(t.x, t.y);
+                    if (2 * c2 + 1 < NCG) v[2 * c2 + 1] = make_float2(t.z, t.w);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) {
+                    v[c] = make_float2(0.f, 0.f);
+                    if (c < ncg) {
+                        const int64_t coil = cg0 + c;
+                        v[c] = (a.layout == GN_COIL_LAST) ? a.in[(int64_t)pa.x * a.nc + coil] : a.in[coil * a.M + pa.x];
+                    }
                 }
             }
             float4 *dv = (float4 *)(s_val + tid * 16);

@@ -309,6 +309,16 @@ __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
     fft_cols_body<MASK>(a, blockIdx.x, blockIdx.y);
 }
 
+// Pass C of a 3-D type 2 (m0 rows in, nf0 rows out at a stride of a whole plane):
+// blocks of twice the threads and kWideCB columns, 256-byte row segments.  Measured
+// faster for this pass only; the other passes keep kFFTColThreads and CB <= GN_FFT_CB.
+constexpr int kFFTColThreadsWide = 2 * kFFTColThreads;
+constexpr int kWideCB = 2 * GN_FFT_CB;
+__global__ __launch_bounds__(kFFTColThreadsWide) void k_fft_cols_wide(ColArgs a)
+{
+    fft_cols_body<0>(a, blockIdx.x, blockIdx.y);
+}
+
 // Batched column passes: items[b] = {argument index, column block, slab, 0}.
 __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols_batch(const ColArgs *__restrict__ args,
                                                                    const int4 *__restrict__ items)
@@ -779,7 +789,18 @@ int pruned_type2_cols(gn_plan_s *p, float sign)
     ColArgs a[2];
     const int n = pruned_col_args(p, true, sign, a);
     for (int k = 0; k < n; ++k) {
-        hipError_t e = launch_cols(a[k], p->stream);
+        hipError_t e;
+        ColArgs &c = a[k];
+        const size_t wide_lds = col_lds(c.L, kWideCB);
+        if (k == n - 1 && p->geo.dim == 3 && c.CB == GN_FFT_CB && wide_lds <= 80 * 1024) {
+            c.CB = kWideCB; // two blocks per CU
+            dim3 grid((unsigned)((c.ncols + c.CB - 1) / c.CB), (unsigned)c.nslabs);
+            allow_lds((const void *)k_fft_cols_wide, wide_lds);
+            hipLaunchKernelGGL(k_fft_cols_wide, grid, dim3(kFFTColThreadsWide), wide_lds, p->stream, c);
+            e = hipGetLastError();
+        } else {
+            e = launch_cols(c, p->stream);
+        }
         if (e != hipSuccess) return hip_fail(e, k + 1 < n ? "FFT pass B" : "FFT pass C");
     }
     return GN_OK;

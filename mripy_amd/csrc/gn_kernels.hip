@@ -1149,35 +1149,47 @@ template <int D>
 __device__ inline void stage_region_pair(float4 *lds, const float2 *__restrict__ g0, const float2 *__restrict__ g1,
                                          bool two, const int64_t *nf, const int *org, int R1, int R2, int RC)
 {
+    // A thread takes cells rr = tid, tid + NT, ... of the region, rr = (r0 * R1 + r1) * R2 + r2
+    // (R2 = 1 in 2-D), and steps its (r0, r1, r2) by NT = (d0 * R1 + d1) * R2 + d2 with
+    // carries instead of dividing for every cell.
+    const int NT = blockDim.x;
     const int R12 = R1 * R2;
-    const float rR12 = 1.0f / (float)R12, rR2 = 1.0f / (float)R2;
-    for (int base = threadIdx.x; base < RC; base += kStageUnroll * blockDim.x) {
+    const int d0 = NT / R12, dr = NT - d0 * R12, d1 = dr / R2, d2 = dr - d1 * R2;
+    const int tid = threadIdx.x;
+    int r0 = tid / R12;
+    int r1 = (tid - r0 * R12) / R2;
+    int r2 = tid - r0 * R12 - r1 * R2;
+    const int nf0 = (int)nf[0], nf1 = (int)nf[1], nf2 = D == 3 ? (int)nf[2] : 1;
+    for (int base = tid; base < RC; base += kStageUnroll * NT) {
         float4 v[kStageUnroll];
 #pragma unroll
         for (int u = 0; u < kStageUnroll; ++u) {
-            const int rr = base + u * blockDim.x;
+            const int rr = base + u * NT;
             if (rr < RC) {
-                const int r0 = fdiv_small(rr, R12, rR12);
-                const int rem = rr - r0 * R12;
-                const int r1 = D == 3 ? fdiv_small(rem, R2, rR2) : rem;
-                const int r2 = D == 3 ? rem - r1 * R2 : 0;
                 int q0 = org[0] + r0;
-                if (q0 >= nf[0]) q0 -= (int)nf[0];
+                if (q0 >= nf0) q0 -= nf0;
                 int q1 = org[1] + r1;
-                if (q1 >= nf[1]) q1 -= (int)nf[1];
-                int64_t gi = (int64_t)q0 * nf[1] + q1;
+                if (q1 >= nf1) q1 -= nf1;
+                int64_t gi = (int64_t)q0 * nf1 + q1;
                 if (D == 3) {
                     int q2 = org[2] + r2;
-                    if (q2 >= nf[2]) q2 -= (int)nf[2];
-                    gi = gi * nf[2] + q2;
+                    if (q2 >= nf2) q2 -= nf2;
+                    gi = gi * nf2 + q2;
                 }
                 const float2 x0 = g0[gi], x1 = two ? g1[gi] : make_float2(0.f, 0.f);
                 v[u] = make_float4(x0.x, x0.y, x1.x, x1.y);
             }
+            r2 += d2;
+            int c = r2 >= R2 ? 1 : 0;
+            r2 -= c * R2;
+            r1 += d1 + c;
+            c = r1 >= R1 ? 1 : 0;
+            r1 -= c * R1;
+            r0 += d0 + c;
         }
 #pragma unroll
         for (int u = 0; u < kStageUnroll; ++u) {
-            const int rr = base + u * blockDim.x;
+            const int rr = base + u * NT;
             if (rr < RC) lds[rr] = v[u];
         }
     }

@@ -115,7 +115,9 @@ int gn_plan_info(const gn_plan *plan, int64_t *nf, int *nspread, double *tau, in
 
 /* Override the tile shapes, tile[6] = spread tile (3 axes) then gather tile (3 axes),
  * in cells; 0 keeps the default.  Invalidates the binned points: call gn_set_points
- * again before gn_execute. */
+ * again before gn_execute.  Spread tiles are at most 32 cells per axis; a gather tile
+ * plus its halo (tile + 2 nspread - 1 cells per axis, 16 bytes per cell: a coil pair)
+ * must fit 160 KiB of LDS, else GN_EINVAL. */
 int gn_plan_set_tiles(gn_plan *plan, const int *tile);
 
 /* Stage timers (replaces the wall-clock utc.timing prints around the gridding and FFT

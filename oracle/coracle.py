@@ -149,6 +149,16 @@ def type1(coords, c, p: O.Params, threads=None, fft_workers=None):
     return out if has_coil else out[..., 0]
 
 
+def forward_grid(Fk, p: O.Params, fft_workers=None):
+    """``nufft_oracle.forward_grid`` with a multi-threaded FFT: the oversampled grid
+    of the type-2 map before the gather, coil-last (*nf, nc) complex128."""
+    F2 = np.asarray(Fk).reshape(p.modes + (-1,)) * O.deapod(p.modes, p.tau)[..., None]
+    grid = O.place(F2, p.nf)
+    axes = tuple(range(p.d))
+    w = fft_workers or max(1, cores() - 1)
+    return _fftn(grid, axes, True, w) * p.G if p.iflag < 0 else _fftn(grid, axes, False, w)
+
+
 def type2(coords, Fk, p: O.Params, threads=None, fft_workers=None):
     """Forward NUFFT through the C gather (same contract as nufft_oracle.type2)."""
     coords = [_f64(x) * p.df for x in coords]

@@ -323,6 +323,15 @@ def adjoint_exact(coords, y, p: Params, fast=False):
     return type1(coords, y, p, fast) * M
 
 
+def forward_grid(Fk, p: Params):
+    """Oversampled grid of the type-2 map before the gather: deapodize, place at
+    ``k mod nf`` and FFT (fft/nufft_func.py:761-771, 1226-1242), coil-last (*nf, nc)."""
+    F2, _ = _as_coil_last(Fk, p.modes)
+    grid = place(F2 * deapod(p.modes, p.tau)[..., None], p.nf)
+    axes = tuple(range(p.d))
+    return np.fft.ifftn(grid, axes=axes) * p.G if p.iflag < 0 else np.fft.fftn(grid, axes=axes)
+
+
 def traj_grad(coords, Fk, w, p: Params):
     """FP64 analytic coordinate gradient of the type-2 map (no reference counterpart).
 
@@ -330,13 +339,16 @@ def traj_grad(coords, Fk, w, p: Params):
     d/dxi_a exp(-(xi_a - h g_a)^2/(4 tau)) = -(xi_a - h g_a)/(2 tau) * exp(...),
     times df (xi = df x mod 2 pi).
     """
+    return traj_grad_on_grid(coords, forward_grid(Fk, p), w, p)
+
+
+def traj_grad_on_grid(coords, grid, w, p: Params):
+    """``traj_grad`` from a precomputed ``forward_grid`` (*nf, nc): sample j's
+    gradient depends only on sample j and the grid, so any subset of the samples
+    can be evaluated on the full grid."""
     coords = [np.asarray(x, dtype=np.float64) * p.df for x in coords]
     M = coords[0].shape[0]
-    F2, _ = _as_coil_last(Fk, p.modes)
-    grid = place(F2 * deapod(p.modes, p.tau)[..., None], p.nf)
-    axes = tuple(range(p.d))
-    grid = np.fft.ifftn(grid, axes=axes) * p.G if p.iflag < 0 else np.fft.fftn(grid, axes=axes)
-    g2 = grid.reshape(p.G, -1)
+    g2 = np.asarray(grid).reshape(p.G, -1)
     w2 = np.asarray(w).reshape(M, -1)
     idx, wt, dw = [], [], []
     for a, x in enumerate(coords):

@@ -186,6 +186,21 @@ def test_cgd2_matches_fp64_oracle():
     assert isinstance(got_np, np.ndarray) and rel(got_np, want) <= TOL
 
 
+def test_cgd2_fletcher_reeves_beta_hand_trace():
+    """The GPU CG on the hand-derived trace of tests/test_solvers_oracle.py:
+    A = diag(1/2, 1/4), b = (1, 1), one iteration after the first step; the
+    un-squared beta = ||g1||/||g0|| puts x at (1.625 + 0.75 beta, 1.12890625 + 0.46875 beta)."""
+    import torch
+    from mripy_amd import solvers as S
+    print("case: CGD2 hand trace", flush=True)
+    a = torch.tensor([0.5, 0.25], dtype=torch.complex64, device="cuda")
+    b = torch.ones(2, dtype=torch.complex64, device="cuda")
+    x = S.prox_l2_Afxnb_CGD2(lambda v: a * v, lambda r: a.conj() * r, b, 1).cpu().numpy()
+    beta = np.sqrt(0.3088531494140625 / 0.7822265625)
+    want = np.array([1.25 + 0.375 + 0.75 * beta, 0.71875 + 0.41015625 + 0.46875 * beta])
+    assert np.abs(x - want).max() < 1e-6, (x, want)
+
+
 def test_cgd_proximal_matches_fp64_oracle():
     from mripy_amd import solvers as S
     print("case: CGD", flush=True)

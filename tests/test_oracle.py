@@ -120,6 +120,26 @@ def test_type2_bruteforce_and_adjointness_2d():
     assert abs(lhs - rhs) / abs(lhs) < 1e-13
 
 
+@pytest.mark.parametrize("iflag,df", [(-1, 1.0), (-1, 1.5), (1, 1.5)])
+def test_type2_bruteforce_sign_and_df(iflag, df):
+    # the iflag < 0 branch (inverse FFT times G, fft/nufft_func.py:768-771) and the df
+    # scaling of the coordinates against the brute-force type-2 sum at eps=1e-15
+    rng = np.random.RandomState(11)
+    M = 2500
+    x, y = 20 * rng.rand(2, M) - 10
+    F = rng.randn(12, 10) + 1j * rng.randn(12, 10)
+    p = O.Params((12, 10), eps=1e-15, iflag=iflag, df=df)
+    got = O.type2([x, y], F, p)
+    assert rel(got, O.nudft_type2([x, y], F, (12, 10), df=df, iflag=iflag)) < 1e-11
+    # 3-D as well, through the NumPy and the C restatements
+    xyz = list(12 * rng.rand(3, 1500) - 6)
+    p3 = O.Params((8, 8, 8), eps=1e-15, iflag=iflag, df=df, max_nspread=6)
+    F3 = rng.randn(8, 8, 8) + 1j * rng.randn(8, 8, 8)
+    want = O.nudft_type2(xyz, F3, (8, 8, 8), df=df, iflag=iflag)
+    assert rel(O.type2(xyz, F3, p3), want) < 1e-6
+    assert rel(C.type2(xyz, F3, p3), want) < 1e-6
+
+
 def test_adjointness_3d_multicoil_rat2():
     rng = np.random.RandomState(3)
     M, nc = 2000, 2

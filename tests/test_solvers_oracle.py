@@ -94,6 +94,30 @@ def test_cgd_proximal_one_newton_step_is_exact():
     assert rel(x, want) < 1e-13
 
 
+# A worked two-step trace of prox_l2_Afxnb_CGD2 (pics/proximal_func.py:314-343) on
+# A = diag(1/2, 1/4), b = (1, 1), Nite = 1, derived by hand:
+#   x0 = A^H b = (0.5, 0.25);  g0 = 2 A^H (A x0 - b) = (-0.75, -0.46875)
+#   trial step alpha = 1 along -g0: x0 - g0 = (1.25, 0.71875) with f = 0.81353759765625
+#   < f(x0) = 1.44140625, so the line search accepts it at once:  x1 = (1.25, 0.71875)
+#   g1 = 2 A^H (A x1 - b) = (-0.375, -0.41015625)
+#   beta = ||g1|| / ||g0|| = sqrt(0.3088531494140625 / 0.7822265625)   (norms, not squares)
+#   s1 = -g1 + beta (-g0);  x1 + s1 has f = 0.41723 < f(x1): accepted, x2 = x1 + s1.
+CG_TRACE_A = np.diag([0.5, 0.25]).astype(np.complex128)
+CG_TRACE_BETA = np.sqrt(0.3088531494140625 / 0.7822265625)
+
+
+def cg_trace_expected(beta):
+    return np.array([1.25 + 0.375 + 0.75 * beta, 0.71875 + 0.41015625 + 0.46875 * beta])
+
+
+def test_cgd2_fletcher_reeves_beta_hand_trace():
+    A, b = CG_TRACE_A, np.ones(2, np.complex128)
+    x = S.prox_l2_Afxnb_CGD2(lambda v: A @ v, lambda r: A.conj().T @ r, b, Nite=1)
+    assert np.abs(x - cg_trace_expected(CG_TRACE_BETA)).max() < 1e-14
+    # the textbook squared ratio ||g1||^2/||g0||^2 lands elsewhere: the trace tells them apart
+    assert np.abs(cg_trace_expected(CG_TRACE_BETA ** 2) - cg_trace_expected(CG_TRACE_BETA)).max() > 0.1
+
+
 def test_cgd2_stops_when_the_line_search_is_exhausted():
     # the un-squared Fletcher-Reeves beta lets the direction turn uphill; the line search
     # then backtracks ls_Nite times and the loop ends (pics/proximal_func.py:331)

@@ -80,7 +80,10 @@ int gn_grid_params(int dim, const int64_t *modes, double eps, int max_nspread, i
 
 /* Create a plan for a dim-D (2 or 3) transform of an image of size modes[0..dim-1]
  * with ncoils coils, oversampling rat and half-width nspread (the window spans
- * 2*nspread taps per axis).  iflag >= 0: type 2 uses e^{-ik.x}; iflag < 0 flips both
+ * 2*nspread taps per axis).  Supported nspread: 2..8 in 3-D (the reference drivers
+ * clamp it to 3..6), 2..16 in 2-D (eps down to about 1e-18 at rat 3; smaller eps
+ * gives GN_EINVAL).  The default gather tile shrinks for wide windows so that tile
+ * plus halo fits the LDS.  iflag >= 0: type 2 uses e^{-ik.x}; iflag < 0 flips both
  * signs.  stream: hipStream_t (NULL = default stream). */
 int gn_plan_create(int dim, const int64_t *modes, int rat, int nspread, int iflag, int64_t ncoils, int layout,
                    void *stream, gn_plan **plan);

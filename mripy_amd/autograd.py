@@ -26,11 +26,8 @@ def _bind(plan, coords):
         torch.is_tensor(p) and p.data_ptr() == coords[i].data_ptr() and p._version == coords[i]._version
         for i, p in enumerate(pts))
     if not same:
-        plan.set_points([coords[i].detach() for i in range(coords.shape[0])], df=plan_df(plan))
-
-
-def plan_df(plan):
-    return getattr(plan, "df", 1.0)
+        # keep the plan's df and density weights (the exact adjoint ignores the weights)
+        plan.set_points([coords[i].detach() for i in range(coords.shape[0])], df=plan.df, dcf=plan._dcf)
 
 
 class _Forward(torch.autograd.Function):

@@ -237,7 +237,9 @@ int plan_create_impl(int dim, const int64_t *modes, int rat, int nspread, int if
     if (!modes) return fail(GN_EINVAL, "modes is null");
     if (rat < 2) return fail(GN_EINVAL, "rat must be >= 2");
     const int max_ns = dim == 3 ? 8 : 16;
-    if (nspread < 2 || nspread > max_ns) return fail(GN_EINVAL, "nspread out of the supported range");
+    if (nspread < 2 || nspread > max_ns)
+        return fail(GN_EINVAL, dim == 3 ? "nspread out of the supported range [2, 8] for 3-D plans"
+                                        : "nspread out of the supported range [2, 16] for 2-D plans (eps >= ~1e-18)");
     if (ncoils < 1) return fail(GN_EINVAL, "ncoils must be >= 1");
     if (layout != GN_COIL_LAST && layout != GN_COIL_MAJOR) return fail(GN_EINVAL, "unknown layout");
     for (int a = 0; a < dim; ++a)

@@ -169,22 +169,8 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Stable counting sort of items by an integer key in [0, K):
-//   cnt[key*C + chunk]  : items of chunk with that key (key-major)
-//   exclusive scan      : offsets; key-major order keeps the chunk order
-//   scatter             : each chunk places its items in index order
-// keys are read as keys[i * stride].
+// Device exclusive scan of ints (local scans, one block over the block sums, add)
 // ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void k_bin_count(int64_t n, const int *__restrict__ keys, int stride,
-                                                   int64_t chunk_len, int C, int *__restrict__ cnt)
-{
-    const int chunk = blockIdx.x;
-    const int64_t beg = (int64_t)chunk * chunk_len;
-    const int64_t end = min(n, beg + chunk_len);
-    for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
-        atomicAdd(&cnt[(int64_t)keys[i * stride] * C + chunk], 1);
-}
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
@@ -264,49 +250,97 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(int *__restrict__ dat
         if (base + k < n) data[base + k] += add;
 }
 
-__global__ __launch_bounds__(256) void k_bin_starts(const int *__restrict__ offs, int64_t K, int C, int64_t n,
-                                                    int *__restrict__ bin_start)
+// ---------------------------------------------------------------------------
+// Digit passes of the stable counting sort.  A key in [0, K) is sorted by
+// ceil(log2 K / kRadixBits) stable counting sorts on successive kRadixBits-wide
+// digits, least significant first.  Each wave owns one chunk of consecutive items:
+//   k_radix_count   : per-chunk digit histogram (LDS), written digit-major
+//                     cnt[digit * C + chunk]
+//   exclusive scan  : cnt -> global start of every (digit, chunk) run; digit-major
+//                     order keeps chunks, hence items, in index order
+//   k_radix_scatter : the wave walks its chunk 64 items at a time; the lanes holding
+//                     the same digit find each other with one ballot per digit bit
+//                     (match mask), rank themselves by lane, and the lowest lane of
+//                     each group advances that digit's running offset in LDS.
+// Every pass is stable, so the result is the stable order by key.
+// ---------------------------------------------------------------------------
+
+constexpr int kRadixBits = 10;
+constexpr int kRadixBins = 1 << kRadixBits;
+constexpr int kRadixWaves = 4;
+
+__global__ __launch_bounds__(64 * kRadixWaves) void k_radix_count(int64_t n, const int *__restrict__ keys, int stride,
+                                                                  int shift, int bins, int64_t chunk_len, int C,
+                                                                  int *__restrict__ cnt)
 {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < K) bin_start[k] = offs[k * C];
-    if (k == K) bin_start[K] = (int)n;
+    __shared__ int hist[kRadixWaves][kRadixBins];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int chunk = blockIdx.x * kRadixWaves + wave;
+    for (int d = lane; d < bins; d += 64) hist[wave][d] = 0;
+    __syncthreads();
+    if (chunk < C) {
+        const int64_t beg = (int64_t)chunk * chunk_len;
+        const int64_t end = min(n, beg + chunk_len);
+        for (int64_t i = beg + lane; i < end; i += 64)
+            atomicAdd(&hist[wave][(keys[i * stride] >> shift) & (bins - 1)], 1);
+    }
+    __syncthreads();
+    if (chunk < C)
+        for (int d = lane; d < bins; d += 64) cnt[(int64_t)d * C + chunk] = hist[wave][d];
 }
 
-// One wave per chunk; items are placed group by group in index order, so
-// equal keys keep their original order (stable, deterministic).
-__global__ __launch_bounds__(64) void k_bin_scatter(int64_t n, const int *__restrict__ keys, int stride,
-                                                    int64_t chunk_len, int C, int *__restrict__ offs,
-                                                    int *__restrict__ sorted)
+// vals_in == nullptr: the values are the item indices (first pass)
+__global__ __launch_bounds__(64 * kRadixWaves) void k_radix_scatter(int64_t n, const int *__restrict__ keys_in,
+                                                                    int stride, const int *__restrict__ vals_in,
+                                                                    int shift, int bits, int64_t chunk_len, int C,
+                                                                    const int *__restrict__ offs,
+                                                                    int *__restrict__ keys_out,
+                                                                    int *__restrict__ vals_out)
 {
-    __shared__ int sk[64];
-    const int lane = threadIdx.x;
-    const int chunk = blockIdx.x;
+    __shared__ int base[kRadixWaves][kRadixBins];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int chunk = blockIdx.x * kRadixWaves + wave;
+    const int bins = 1 << bits;
+    if (chunk < C)
+        for (int d = lane; d < bins; d += 64) base[wave][d] = offs[(int64_t)d * C + chunk];
+    __syncthreads();
+    if (chunk >= C) return;
     const int64_t beg = (int64_t)chunk * chunk_len;
     const int64_t end = min(n, beg + chunk_len);
+    const unsigned long long lt = (1ull << lane) - 1ull;
     for (int64_t g = beg; g < end; g += 64) {
         const int64_t i = g + lane;
         const bool valid = i < end;
-        const int key = valid ? keys[i * stride] : -1;
-        sk[lane] = key;
-        __syncthreads();
-        int rank = 0;
-        bool last = true;
-        for (int l = 0; l < 64; ++l) {
-            const int kl = sk[l];
-            if (kl == key) {
-                if (l < lane) ++rank;
-                if (l > lane) last = false;
-            }
+        const int key = valid ? keys_in[i * stride] : 0;
+        const int val = valid ? (vals_in ? vals_in[i] : (int)i) : 0;
+        const int d = (key >> shift) & (bins - 1);
+        unsigned long long m = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool on = (d >> b) & 1;
+            const unsigned long long bb = __ballot(on);
+            m &= on ? bb : ~bb;
         }
-        int base = 0;
+        const int rank = __popcll(m & lt);
+        const int b0 = base[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) base[wave][d] = b0 + __popcll(m);
+        __builtin_amdgcn_wave_barrier();
         if (valid) {
-            base = offs[(int64_t)key * C + chunk];
-            sorted[base + rank] = (int)i;
+            keys_out[b0 + rank] = key;
+            vals_out[b0 + rank] = val;
         }
-        __syncthreads();
-        if (valid && last) offs[(int64_t)key * C + chunk] = base + rank + 1;
-        __syncthreads();
     }
+}
+
+// bin_start[k] = first position of key k in the sorted keys (n where none), k = 0..K
+__global__ __launch_bounds__(256) void k_bin_starts_sorted(int64_t n, const int *__restrict__ skeys, int64_t K,
+                                                           int *__restrict__ bin_start)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const int64_t kprev = i == 0 ? -1 : skeys[i - 1];
+    const int64_t knext = i == n ? K : skeys[i];
+    for (int64_t k = kprev + 1; k <= knext; ++k) bin_start[k] = (int)i;
 }
 
 __global__ __launch_bounds__(256) void k_make_records(int64_t M, const int *__restrict__ sorted,
@@ -1595,37 +1629,61 @@ hipError_t device_scan(int *data, int64_t n, int *bsum, hipStream_t st)
     return hipGetLastError();
 }
 
-// Stable counting sort of n items by keys[i*stride] in [0,K).  Writes the sorted
-// item indices and the K+1 bucket offsets.
+// Stable counting sort of n items by keys[i*stride] in [0,K): writes the sorted item
+// indices and the K+1 bucket offsets.  Asynchronous on st; the temporary buffers
+// belong to sc, which the caller keeps alive until the stream has consumed them.
 int stable_sort_by_key(int64_t n, const int *keys, int stride, int64_t K, int *sorted, int *bin_start,
-                       hipStream_t st)
+                       hipStream_t st, Scratch &sc)
 {
-    Scratch sc;
-    int64_t C = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)(8 << 20) / std::max<int64_t>(K, 1)));
-    int64_t chunk_len = (n + C - 1) / C;
-    chunk_len = ((chunk_len + 63) / 64) * 64;
-    C = (n + chunk_len - 1) / chunk_len;
-    const int64_t nKC = K * C;
-    const int64_t nb = (nKC + kScanTile - 1) / kScanTile;
-    int *cnt = nullptr, *bsum = nullptr;
     int rc;
-    if ((rc = sc.alloc(&cnt, (size_t)nKC, "sort counts")) || (rc = sc.alloc(&bsum, (size_t)nb + 1, "sort scan")))
+    int kbits = 1;
+    while (kbits < 31 && ((int64_t)1 << kbits) < K) ++kbits;
+    const int npass = (kbits + kRadixBits - 1) / kRadixBits;
+    // chunks: about 4096 waves over the GPU, 256..4096 items each
+    int64_t chunk_len = (n + 4095) / 4096;
+    chunk_len = std::min<int64_t>(4096, std::max<int64_t>(256, ((chunk_len + 63) / 64) * 64));
+    const int64_t C = std::max<int64_t>(1, (n + chunk_len - 1) / chunk_len);
+    const int64_t nKC = (int64_t)kRadixBins * C;
+    const int64_t nb = (nKC + kScanTile - 1) / kScanTile;
+    int *cnt = nullptr, *bsum = nullptr, *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    if ((rc = sc.alloc(&cnt, (size_t)nKC, "sort counts")) || (rc = sc.alloc(&bsum, (size_t)nb + 1, "sort scan")) ||
+        (rc = sc.alloc(&k0, (size_t)n, "sort keys")) || (rc = sc.alloc(&k1, (size_t)n, "sort keys")) ||
+        (rc = sc.alloc(&v0, (size_t)n, "sort values")))
         return rc;
-    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(int) * nKC, st);
+    v1 = sorted;
+    const unsigned nblk = (unsigned)((C + kRadixWaves - 1) / kRadixWaves);
+    const int *kin = keys;
+    const int *vin = nullptr;
+    int kstride = stride;
+    // ping-pong so that the last pass writes its values into `sorted`
+    int *kout[2] = {k0, k1};
+    int *vout[2] = {v0, v1};
+    int shift = 0;
+    hipError_t e = hipSuccess;
+    for (int ps = 0; ps < npass && e == hipSuccess; ++ps) {
+        const int bits = std::min(kRadixBits, kbits - shift);
+        const int bins = 1 << bits;
+        const int o = (npass - 1 - ps) % 2 == 0 ? 1 : 0; // the last pass writes slot 1 (sorted)
+        const int64_t nbins = (int64_t)bins * C;
+        hipLaunchKernelGGL(k_radix_count, dim3(nblk), dim3(64 * kRadixWaves), 0, st, n, kin, kstride, shift, bins,
+                           chunk_len, (int)C, cnt);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = device_scan(cnt, nbins, bsum, st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(64 * kRadixWaves), 0, st, n, kin, kstride, vin, shift,
+                               bits, chunk_len, (int)C, cnt, kout[o], vout[o]);
+            e = hipGetLastError();
+        }
+        kin = kout[o];
+        vin = vout[o];
+        kstride = 1;
+        shift += bits;
+    }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_bin_count, dim3((unsigned)C), dim3(256), 0, st, n, keys, stride, chunk_len, (int)C, cnt);
+        hipLaunchKernelGGL(k_bin_starts_sorted, dim3(grid_blocks(n + 1, 256, INT32_MAX)), dim3(256), 0, st, n, kin, K,
+                           bin_start);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = device_scan(cnt, nKC, bsum, st);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_bin_starts, dim3(grid_blocks(K + 1, 256, INT32_MAX)), dim3(256), 0, st, cnt, K, (int)C,
-                           n, bin_start);
-        hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)C), dim3(64), 0, st, n, keys, stride, chunk_len, (int)C, cnt,
-                           sorted);
-        e = hipGetLastError();
-    }
-    // keep the scratch alive until the stream has consumed it
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "stable sort");
     return GN_OK;
 }
@@ -1653,8 +1711,20 @@ void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
         gtile[1] = 24;
         gtile[2] = 1;
     }
-    (void)ns;
-    (void)nf;
+    // wide windows: shrink the gather tile (largest axis first) until tile + halo of a
+    // coil pair fits the 160 KiB of LDS
+    auto staged = [&]() {
+        int64_t r = kGatherCellBytes;
+        for (int a = 0; a < dim; ++a) r *= std::min<int64_t>(gtile[a] + 2 * ns - 1, nf[a]);
+        return r;
+    };
+    while (staged() > 160 * 1024) {
+        int a = 0;
+        for (int b = 1; b < dim; ++b)
+            if (gtile[b] > gtile[a]) a = b;
+        if (gtile[a] == 1) break;
+        --gtile[a];
+    }
 }
 
 // Orders the records of one gather work item for conflict-free LDS reads in the
@@ -1784,14 +1854,10 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         hipLaunchKernelGGL(k_prep, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, pa);
         GN_HIP(hipGetLastError());
         // gather-tile sort (stable) -> records in tile order
-        if ((rc = stable_sort_by_key(M, &cell[0].w, 4, Kg, sorted, R.bin_start, st))) return rc;
+        if ((rc = stable_sort_by_key(M, &cell[0].w, 4, Kg, sorted, R.bin_start, st, sc))) return rc;
         hipLaunchKernelGGL(k_make_records, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, sorted, cell,
                            frac, dcf, R.cell, R.frac);
         GN_HIP(hipGetLastError());
-        int herr = 0;
-        GN_HIP(hipMemcpyAsync(&herr, p->err_flag, sizeof(int), hipMemcpyDeviceToHost, st));
-        GN_HIP(hipStreamSynchronize(st));
-        if (herr) return fail(GN_EINVAL, "non-finite sample coordinate");
 
         // (spread tile, sample) pair records, generated in record order
         TileGeom tg;
@@ -1807,13 +1873,16 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         if ((rc = sc.alloc(&pcnt, (size_t)M, "pair counts")) || (rc = sc.alloc(&pbsum, (size_t)nbp + 1, "pair scan")))
             return rc;
         hipLaunchKernelGGL(k_pair_count, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, R.cell, tg, pcnt);
-        int last_cnt = 0;
-        GN_HIP(hipMemcpyAsync(&last_cnt, pcnt + (M - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipGetLastError());
+        // the pair total is the scan's grand total (pbsum[nb]); one host round trip
+        // returns it together with the coordinate error flag
         GN_HIP(device_scan(pcnt, M, pbsum, st));
-        int last_off = 0;
-        GN_HIP(hipMemcpyAsync(&last_off, pcnt + (M - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        int hv[2] = {0, 0};
+        GN_HIP(hipMemcpyAsync(&hv[0], pbsum + nbp, sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipMemcpyAsync(&hv[1], p->err_flag, sizeof(int), hipMemcpyDeviceToHost, st));
         GN_HIP(hipStreamSynchronize(st));
-        const int64_t P = (int64_t)last_off + last_cnt;
+        if (hv[1]) return fail(GN_EINVAL, "non-finite sample coordinate");
+        const int64_t P = (int64_t)(unsigned)hv[0];
         if (P > (int64_t)INT32_MAX) return fail(GN_EINVAL, "too many (tile, sample) pairs");
         int *pkey = nullptr, *psorted = nullptr, *pstart = nullptr;
         int4 *ta = nullptr, *tb = nullptr;
@@ -1824,7 +1893,7 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         hipLaunchKernelGGL(k_pair_emit, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, R.cell, R.frac, tg,
                            pcnt, pkey, ta, tb);
         GN_HIP(hipGetLastError());
-        if ((rc = stable_sort_by_key(P, pkey, 1, K, psorted, pstart, st))) return rc;
+        if ((rc = stable_sort_by_key(P, pkey, 1, K, psorted, pstart, st, sc))) return rc;
         if (P > R.pairs_cap) {
             if ((rc = dev_alloc(&R.pair_a, (size_t)P, "pairs")) || (rc = dev_alloc(&R.pair_b, (size_t)P, "pairs")))
                 return rc;

@@ -20,6 +20,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from .operators import _fingerprint
 from .plan import Plan
 
 _plan_cache = {}
@@ -69,7 +70,11 @@ def _run(kind, coords, data, modes, df, eps, iflag, rat, nspread, dcf=None):
         trail = tuple(np.shape(data))[d:]
     nc = int(np.prod(trail)) if trail else 1
     P = _get_plan(modes, nc, eps, iflag, rat, nspread, dev)
-    P.set_points(list(coords), df=df, dcf=dcf)
+    # re-bin only when the coordinates, df or weights differ from the plan's last binning
+    key = tuple(_fingerprint(c) for c in coords) + (float(df), _fingerprint(dcf))
+    if getattr(P, "_fn_key", None) != key:
+        P.set_points(list(coords), df=df, dcf=dcf)
+        P._fn_key = key
     t = torch.as_tensor(data)
     if kind == 1:
         out = P.type1(t.reshape(M, nc))

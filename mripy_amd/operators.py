@@ -42,6 +42,25 @@ def _flat(a):
     return a.reshape(-1) if torch.is_tensor(a) else np.asarray(a).flatten()
 
 
+def _fingerprint(a):
+    """Identity of an input's current contents.  Torch tensors: storage pointer and
+    version counter (every in-place update bumps it, also through views).  numpy
+    arrays: a 64-bit hash of the bytes, so in-place edits of the trajectory or the
+    density weights are seen."""
+    if a is None:
+        return None
+    if torch.is_tensor(a):
+        return ("t", a.data_ptr(), a._version, tuple(a.shape), str(a.dtype), str(a.device))
+    a = np.ascontiguousarray(a)
+    try:
+        import xxhash
+        h = xxhash.xxh3_64_intdigest(memoryview(a).cast("B"))
+    except ImportError:
+        import hashlib
+        h = hashlib.blake2b(memoryview(a).cast("B"), digest_size=8).hexdigest()
+    return ("n", a.shape, str(a.dtype), h)
+
+
 class _NUFFTnd:
     """Shared implementation of the d-dimensional operator."""
 
@@ -117,7 +136,7 @@ class _NUFFTnd:
                               device=device, tiles=self.tiles, gather_tiles=self.gather_tiles)
             self._plan_key = key
             self._points_key = None
-        pk = tuple(id(v) for v in self._coords()) + (id(self.dcf),)
+        pk = tuple(_fingerprint(v) for v in self._coords()) + (_fingerprint(self.dcf), self.df)
         if self._points_key != pk:
             self._plan.set_points(self._coords(), df=self.df, dcf=self.dcf)
             self._points_key = pk

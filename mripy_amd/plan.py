@@ -76,6 +76,7 @@ class Plan:
         self.tau = tau.value
         self.M = None
         self._dcf = None
+        self.df = 1.0
 
     # ------------------------------------------------------------------ setup
     def set_tiles(self, tiles=None, gather_tiles=None):
@@ -131,6 +132,7 @@ class Plan:
                                             float(df), d.data_ptr() if d is not None else None))
         self.M = M
         self._dcf = d
+        self.df = float(df)
         self._points = [c for c in coords]
         self._bound = (ts, f64, float(df), d)   # device copies, for re-binning by autotune
         return self

@@ -56,6 +56,12 @@ extern "C" {
 #define GN_TYPE2 2   /* y = (1/G) gather(fft(pad((pi/tau)^(d/2) e^{tau|k|^2} image)))       */
 #define GN_ADJOINT 3 /* exact adjoint of GN_TYPE2: M * GN_TYPE1 without dcf                  */
 #define GN_TYPE21 21 /* GN_TYPE1(GN_TYPE2(image)) with dcf                                   */
+/* SENSE kinds (maps set by gn_plan_set_sense): the image is one coil-less image x of
+ * modes[0..dim-1] points; the samples keep their nc coils.                             */
+#define GN_SENSE_TYPE2 32    /* y = GN_TYPE2(S x): coil c's image is x * S_c                  */
+#define GN_SENSE_TYPE1 31    /* x = sum_c conj(S_c) GN_TYPE1(y)_c (with dcf)                  */
+#define GN_SENSE_ADJOINT 33  /* exact adjoint of GN_SENSE_TYPE2 (no dcf, no 1/M)              */
+#define GN_SENSE_TYPE21 321  /* GN_SENSE_TYPE1(GN_SENSE_TYPE2(x)) with dcf                    */
 
 /* FFT engines for gn_plan_set_fft_engine */
 #define GN_FFT_AUTO 0   /* pruned in-place passes fused with pad/crop when every nf is a product of
@@ -98,9 +104,21 @@ int gn_plan_set_stream(gn_plan *plan, void *stream);
 int gn_set_points(gn_plan *plan, int64_t M, const void *x, const void *y, const void *z, int coord_f64, double df,
                   const float *dcf);
 
-/* Run one transform (kind = GN_TYPE1/GN_TYPE2/GN_ADJOINT/GN_TYPE21) on the plan's
- * stream.  in/out: device complex64 buffers in the plan's layout.  Asynchronous. */
+/* Run one transform (kind = GN_TYPE1/GN_TYPE2/GN_ADJOINT/GN_TYPE21 or a GN_SENSE_*
+ * kind) on the plan's stream.  in/out: device complex64 buffers in the plan's layout.
+ * Asynchronous. */
 int gn_execute(gn_plan *plan, int kind, const void *in, void *out);
+
+/* SENSE coil sensitivities for the GN_SENSE_* kinds (replaces the composition
+ * joint2operators(espirit(S), NUFFT3d_cuda), pics/operators_class.py:437-472,496-514:
+ * forward = NUFFT.forward(x * S), backward = sum_c conj(S_c) NUFFT.backward(y)_c with
+ * keepdims, forward_backward = espirit.backward(NUFFT.forward_backward(espirit.forward(x)))).
+ * maps: device complex64, the plan's image layout with nc coils (coil-last
+ * (m0,m1[,m2],nc) or coil-major); copied into the plan.  NULL detaches them.
+ * The expansion x * S_c is fused into the first FFT pass and the combine
+ * sum_c conj(S_c) into the last one (one block holds all coils of an image line when
+ * they fit the LDS; otherwise the coil images go through a plan buffer). */
+int gn_plan_set_sense(gn_plan *plan, const void *maps);
 
 /* Gradient of the forward map with respect to the sample coordinates
  * (no counterpart in the reference; for autograd): for image f and complex

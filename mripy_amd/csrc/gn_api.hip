@@ -358,6 +358,26 @@ int gn_plan_set_tiles(gn_plan *p, const int *tile)
     return GN_OK;
 }
 
+int gn_plan_set_sense(gn_plan *p, const void *maps)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    if (!maps) {
+        hipFree(p->sens);
+        hipFree(p->sbuf);
+        p->sens = p->sbuf = nullptr;
+        return GN_OK;
+    }
+    const size_t bytes = (size_t)(p->geo.N * p->nc) * sizeof(float2);
+    if (!p->sens && hipMalloc((void **)&p->sens, bytes) != hipSuccess) {
+        p->sens = nullptr;
+        return fail(GN_ENOMEM, "hipMalloc failed for the SENSE maps");
+    }
+    hipError_t e = hipMemcpyAsync(p->sens, maps, bytes, hipMemcpyDeviceToDevice, p->stream);
+    if (e != hipSuccess) return hip_fail(e, "SENSE maps copy");
+    return GN_OK;
+}
+
 int gn_set_points(gn_plan *p, int64_t M, const void *x, const void *y, const void *z, int coord_f64, double df,
                   const float *dcf)
 {
@@ -381,8 +401,19 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
     g_err.clear();
     if (!p) return fail(GN_EINVAL, "plan is null");
     if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called before gn_execute");
-    if (kind != GN_TYPE1 && kind != GN_TYPE2 && kind != GN_ADJOINT && kind != GN_TYPE21)
+    const bool sense = kind == GN_SENSE_TYPE1 || kind == GN_SENSE_TYPE2 || kind == GN_SENSE_ADJOINT ||
+                       kind == GN_SENSE_TYPE21;
+    if (kind != GN_TYPE1 && kind != GN_TYPE2 && kind != GN_ADJOINT && kind != GN_TYPE21 && !sense)
         return fail(GN_EINVAL, "unknown transform kind");
+    if (sense && !p->sens) return fail(GN_ESTATE, "gn_plan_set_sense must be called before a SENSE transform");
+    if (sense) {
+        switch (kind) {
+        case GN_SENSE_TYPE1: kind = GN_TYPE1; break;
+        case GN_SENSE_TYPE2: kind = GN_TYPE2; break;
+        case GN_SENSE_ADJOINT: kind = GN_ADJOINT; break;
+        default: kind = GN_TYPE21; break;
+        }
+    }
     const Geometry &g = p->geo;
     const bool empty_in = (kind == GN_TYPE2 || kind == GN_TYPE21) ? false : p->M == 0;
     if ((!in && !empty_in) || (!out && !(kind == GN_TYPE2 && p->M == 0)))
@@ -405,10 +436,13 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
         const float scale = (float)(c / (double)g.G);
         int r;
         if (custom) {
-            r = timed(p, GN_STAGE_PAD, [&] { return pruned_type2_rows(p, (const float2 *)img, scale, sign2); });
+            r = timed(p, GN_STAGE_PAD, [&] {
+                return sense ? pruned_type2_rows_sense(p, (const float2 *)img, scale, sign2)
+                             : pruned_type2_rows(p, (const float2 *)img, scale, sign2);
+            });
             if (!r) r = timed(p, GN_STAGE_FFT2, [&] { return pruned_type2_cols(p, sign2); });
         } else {
-            r = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)img, scale); });
+            r = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)img, scale, sense); });
             if (!r) r = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, fwd_dir_type2); });
         }
         if (!r) r = timed(p, GN_STAGE_GATHER, [&] { return launch_gather(p, (float2 *)samples); });
@@ -419,10 +453,25 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
         int r = timed(p, GN_STAGE_SPREAD, [&] { return launch_spread(p, (const float2 *)samples, dcf ? 1 : 0); });
         if (custom) {
             if (!r) r = timed(p, GN_STAGE_FFT1, [&] { return pruned_type1_cols(p, -sign2); });
-            if (!r) r = timed(p, GN_STAGE_CROP, [&] { return pruned_type1_rows(p, (float2 *)img, scale, -sign2); });
+            if (!r && sense && !sense_rows_fit(p)) {
+                // too many coils for one block: coil images into the plan's buffer, then combine
+                if (!p->sbuf && hipMalloc((void **)&p->sbuf, (size_t)(g.N * p->nc) * sizeof(float2)) != hipSuccess) {
+                    p->sbuf = nullptr;
+                    return fail(GN_ENOMEM, "hipMalloc failed for the SENSE coil images");
+                }
+                r = timed(p, GN_STAGE_CROP, [&] {
+                    int q = pruned_type1_rows(p, p->sbuf, scale, -sign2);
+                    return q ? q : launch_sense_combine(p, p->sbuf, (float2 *)img);
+                });
+            } else if (!r) {
+                r = timed(p, GN_STAGE_CROP, [&] {
+                    return sense ? pruned_type1_rows_sense(p, (float2 *)img, scale, -sign2)
+                                 : pruned_type1_rows(p, (float2 *)img, scale, -sign2);
+                });
+            }
         } else {
             if (!r) r = timed(p, GN_STAGE_FFT1, [&] { return run_fft(p, !fwd_dir_type2); });
-            if (!r) r = timed(p, GN_STAGE_CROP, [&] { return launch_crop(p, (float2 *)img, scale); });
+            if (!r) r = timed(p, GN_STAGE_CROP, [&] { return launch_crop(p, (float2 *)img, scale, sense); });
         }
         return r;
     };
@@ -583,6 +632,8 @@ int gn_plan_destroy(gn_plan *p)
     hipFree(p->rec.occ_mask);
     hipFree(p->grid);
     hipFree(p->ybuf);
+    hipFree(p->sens);
+    hipFree(p->sbuf);
     hipFree(p->deap);
     hipFree(p->err_flag);
     delete p;

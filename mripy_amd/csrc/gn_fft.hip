@@ -386,9 +386,17 @@ __device__ __forceinline__ void fft_rows_pad_body(const RowArgs &a, int64_t line
             }
             nn[u] = n;
             cc[u] = c;
-            if (e < total)
-                v[u] = a.layout == GN_COIL_LAST ? a.img_in[(line * a.m + n) * a.nc + c0 + c]
-                                                : a.img_in[(c0 + c) * a.nlines * a.m + line * a.m + n];
+            if (e < total) {
+                if (a.sens) { // SENSE: coil c's line is x * S_c
+                    const float2 x = a.img_in[line * a.m + n];
+                    const float2 sc = a.layout == GN_COIL_LAST ? a.sens[(line * a.m + n) * a.nc + c0 + c]
+                                                               : a.sens[(c0 + c) * a.nlines * a.m + line * a.m + n];
+                    v[u] = cmul(x, sc);
+                } else {
+                    v[u] = a.layout == GN_COIL_LAST ? a.img_in[(line * a.m + n) * a.nc + c0 + c]
+                                                    : a.img_in[(c0 + c) * a.nlines * a.m + line * a.m + n];
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
@@ -458,6 +466,23 @@ __device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t lin
     const float d01 = a.scale * a.deap_a[n0] * (a.deap_b ? a.deap_b[n1] : 1.0f);
     const int half = a.m / 2;
     const bool cl = a.layout == GN_COIL_LAST;
+    if (a.sens) { // SENSE: the block holds every coil (ncl == nc): sum_c conj(S_c) * image_c
+        for (int n = threadIdx.x; n < a.m; n += blockDim.x) {
+            int p = n - half;
+            if (p < 0) p += a.L;
+            const float d = d01 * a.deap_last[n];
+            float2 acc = make_float2(0.f, 0.f);
+            for (int c = 0; c < ncl; ++c) {
+                const float2 v = buf[c * ls + p];
+                const float2 o = make_float2(v.x * d, v.y * d);
+                const float2 sc = cl ? a.sens[(line * a.m + n) * a.nc + c] : a.sens[c * a.nlines * a.m + line * a.m + n];
+                acc.x += sc.x * o.x + sc.y * o.y;
+                acc.y += sc.x * o.y - sc.y * o.x;
+            }
+            a.img_out[line * a.m + n] = acc;
+        }
+        return;
+    }
     const int iw = img_pairs(a, a.img_out, c0, ncl) ? 1 : 0;
     auto get = [&](int n, int c) {
         int p = n - half;
@@ -756,6 +781,44 @@ void fft_axis_free(FFTAxis *ax)
     hipFree(ax->pinv);
     ax->tw = nullptr;
     ax->pinv = nullptr;
+}
+
+// SENSE pass A: coil-less image x -> leading grid rows of x * S_c for every coil.
+int pruned_type2_rows_sense(gn_plan_s *p, const float2 *image, float scale, float sign)
+{
+    const Geometry &g = p->geo;
+    const FFTAxis &az = p->fft_ax[g.dim - 1];
+    RowArgs ra = row_args(p, az, sign, scale);
+    ra.img_in = image;
+    ra.sens = p->sens;
+    dim3 ga((unsigned)ra.nlines, (unsigned)((p->nc + ra.CL - 1) / ra.CL));
+    hipLaunchKernelGGL(k_fft_rows_pad, ga, dim3(kFFTThreads), row_lds(az.L, ra.CL), p->stream, ra);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GN_OK : hip_fail(e, "FFT pass A (SENSE)");
+}
+
+// SENSE pass A': every coil of a line in one block, combined with conj(S_c) into x.
+// GN_EINVAL when the coils do not fit the LDS of one block (the caller then combines
+// separately).
+int pruned_type1_rows_sense(gn_plan_s *p, float2 *image, float scale, float sign)
+{
+    const Geometry &g = p->geo;
+    const FFTAxis &az = p->fft_ax[g.dim - 1];
+    RowArgs ra = row_args(p, az, sign, scale);
+    const size_t lds = row_lds(az.L, (int)p->nc);
+    if (p->nc > 1024 || lds > 160 * 1024) return fail(GN_EINVAL, "SENSE combine: coils do not fit one block");
+    ra.CL = (int)p->nc;
+    ra.img_out = image;
+    ra.sens = p->sens;
+    allow_lds((const void *)k_fft_rows_crop, lds);
+    hipLaunchKernelGGL(k_fft_rows_crop, dim3((unsigned)ra.nlines, 1), dim3(kFFTThreads), lds, p->stream, ra);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GN_OK : hip_fail(e, "FFT pass A' (SENSE)");
+}
+
+bool sense_rows_fit(const gn_plan_s *p)
+{
+    return p->nc <= 1024 && row_lds(p->fft_ax[p->geo.dim - 1].L, (int)p->nc) <= 160 * 1024;
 }
 
 // Type 2, pass A: image -> leading grid rows (pad, deapodize, scale, FFT along the last axis).

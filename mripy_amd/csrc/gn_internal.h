@@ -90,6 +90,10 @@ struct RowArgs {
     Stages st;
     const float2 *tw;
     const int *pinv;
+    // SENSE maps (same layout as the coil images) or null.  Type 2 (pad): the image is
+    // one coil-less image x and coil c's input is x * S_c.  Type 1 (crop): the block
+    // holds every coil of its line and writes sum_c conj(S_c) * image_c.
+    const float2 *sens = nullptr;
 };
 
 // Sample records in tile-sorted order: start cell of the window on each axis
@@ -202,6 +206,9 @@ struct PrepostArgs {
     const float *deap0, *deap1, *deap2;
     const float2 *src; // batched launches: this instance's input
     float2 *dst;       // and output
+    // SENSE maps (same layout as the coil images) or null: pad reads one coil-less
+    // image x and writes x * S_c per coil; crop writes sum_c conj(S_c) * image_c
+    const float2 *sens = nullptr;
 };
 
 } // namespace gn
@@ -225,6 +232,8 @@ struct gn_plan_s {
 
     float2 *grid = nullptr;   // nc * G oversampled grids, coil-major
     float2 *ybuf = nullptr;   // nc * M intermediate samples (type 21)
+    float2 *sens = nullptr;   // N * nc SENSE maps in the plan's layout (gn_plan_set_sense)
+    float2 *sbuf = nullptr;   // N * nc coil images of a SENSE combine that does not fit one block
     int64_t ybuf_cap = 0;
     float *deap = nullptr;    // concatenated per-axis exp(tau k^2) tables
     int *err_flag = nullptr;  // device flag set by set_points on bad coordinates
@@ -263,9 +272,10 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
                  const float *dcf);
 int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf);
 int launch_gather(gn_plan_s *p, float2 *samples);
-int launch_pad(gn_plan_s *p, const float2 *image, float scale);
-int launch_crop(gn_plan_s *p, float2 *image, float scale);
+int launch_pad(gn_plan_s *p, const float2 *image, float scale, bool sense = false);
+int launch_crop(gn_plan_s *p, float2 *image, float scale, bool sense = false);
 int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out);
+int launch_sense_combine(gn_plan_s *p, const float2 *coil_images, float2 *image);
 void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile);
 
 // fft.hip
@@ -276,6 +286,10 @@ int pruned_type2_rows(gn_plan_s *p, const float2 *image, float scale, float sign
 int pruned_type2_cols(gn_plan_s *p, float sign);
 int pruned_type1_cols(gn_plan_s *p, float sign);
 int pruned_type1_rows(gn_plan_s *p, float2 *image, float scale, float sign);
+// SENSE variants of the row passes (p->sens set): coil-less image in / out
+int pruned_type2_rows_sense(gn_plan_s *p, const float2 *image, float scale, float sign);
+int pruned_type1_rows_sense(gn_plan_s *p, float2 *image, float scale, float sign);
+bool sense_rows_fit(const gn_plan_s *p);
 // pieces of the pruned passes for the batched launch (batch.hip)
 size_t col_lds(int L, int CB);
 size_t row_lds(int L, int CL);

@@ -1504,6 +1504,15 @@ __device__ __forceinline__ void pad_body(const PrepostArgs &a, const float2 *__r
             d *= a.deap2[n2];
             n = n * a.modes[2] + n2;
         }
+        if (a.sens) { // SENSE: coil c's image is x * S_c
+            const float2 x = img[n];
+            for (int64_t c = 0; c < a.nc; ++c) {
+                const float2 s = (a.layout == GN_COIL_LAST) ? a.sens[n * a.nc + c] : a.sens[c * a.N + n];
+                const float2 v = make_float2(x.x * s.x - x.y * s.y, x.x * s.y + x.y * s.x);
+                grid[c * a.G + gidx] = make_float2(v.x * d, v.y * d);
+            }
+            continue;
+        }
         for (int64_t c = 0; c < a.nc; ++c) {
             const float2 v = (a.layout == GN_COIL_LAST) ? img[n * a.nc + c] : img[c * a.N + n];
             grid[c * a.G + gidx] = make_float2(v.x * d, v.y * d);
@@ -1551,6 +1560,18 @@ __device__ __forceinline__ void crop_body(const PrepostArgs &a, const float2 *__
             if (i2 < 0) i2 += a.nf[2];
             gi = gi * a.nf[2] + i2;
             d *= a.deap2[n2];
+        }
+        if (a.sens) { // SENSE: sum_c conj(S_c) * image_c, coils in order
+            float2 acc = make_float2(0.f, 0.f);
+            for (int64_t c = 0; c < a.nc; ++c) {
+                const float2 v = grid[c * a.G + gi];
+                const float2 o = make_float2(v.x * d, v.y * d);
+                const float2 s = (a.layout == GN_COIL_LAST) ? a.sens[n * a.nc + c] : a.sens[c * a.N + n];
+                acc.x += s.x * o.x + s.y * o.y;
+                acc.y += s.x * o.y - s.y * o.x;
+            }
+            img[n] = acc;
+            continue;
         }
         for (int64_t c = 0; c < a.nc; ++c) {
             const float2 v = grid[c * a.G + gi];
@@ -2291,9 +2312,10 @@ int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out)
 
 PrepostArgs make_prepost(const gn_plan_s *p, float scale) { return make_prepost_impl(p, scale); }
 
-int launch_pad(gn_plan_s *p, const float2 *image, float scale)
+int launch_pad(gn_plan_s *p, const float2 *image, float scale, bool sense)
 {
     PrepostArgs a = make_prepost(p, scale);
+    if (sense) a.sens = p->sens;
     dim3 grid(grid_blocks(p->geo.G, 256, 1 << 16));
     if (p->geo.dim == 3)
         hipLaunchKernelGGL(k_pad<3>, grid, dim3(256), 0, p->stream, a, image, p->grid);
@@ -2304,9 +2326,10 @@ int launch_pad(gn_plan_s *p, const float2 *image, float scale)
     return GN_OK;
 }
 
-int launch_crop(gn_plan_s *p, float2 *image, float scale)
+int launch_crop(gn_plan_s *p, float2 *image, float scale, bool sense)
 {
     PrepostArgs a = make_prepost(p, scale);
+    if (sense) a.sens = p->sens;
     dim3 grid(grid_blocks(p->geo.N, 256, 1 << 16));
     if (p->geo.dim == 3)
         hipLaunchKernelGGL(k_crop<3>, grid, dim3(256), 0, p->stream, a, p->grid, image);
@@ -2315,6 +2338,32 @@ int launch_crop(gn_plan_s *p, float2 *image, float scale)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "crop launch");
     return GN_OK;
+}
+
+// x[n] = sum_c conj(S[n, c]) * img[n, c] (coils in order), for SENSE combines whose
+// coils do not fit one row-pass block
+__global__ __launch_bounds__(256) void k_sense_combine(int64_t N, int64_t nc, int layout, const float2 *__restrict__ sens,
+                                                       const float2 *__restrict__ img, float2 *__restrict__ x)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int64_t c = 0; c < nc; ++c) {
+            const int64_t i = layout == GN_COIL_LAST ? n * nc + c : c * N + n;
+            const float2 s = sens[i], o = img[i];
+            acc.x += s.x * o.x + s.y * o.y;
+            acc.y += s.x * o.y - s.y * o.x;
+        }
+        x[n] = acc;
+    }
+}
+
+int launch_sense_combine(gn_plan_s *p, const float2 *coil_images, float2 *image)
+{
+    hipLaunchKernelGGL(k_sense_combine, dim3(grid_blocks(p->geo.N, 256, 1 << 16)), dim3(256), 0, p->stream, p->geo.N,
+                       p->nc, p->layout, p->sens, coil_images, image);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GN_OK : hip_fail(e, "SENSE combine");
 }
 
 // ---------------------------------------------------------------------------

@@ -185,6 +185,30 @@ class _NUFFTnd:
         out = P.adjoint(torch.as_tensor(kdata).reshape(P.sample_shape()))
         return self._out(out, kdata, tuple(self.im_shape[:self.dim]) + trail)
 
+    def _sense_apply(self, kind, maps, x):
+        """Fused SENSE transforms for joint2operators(espirit(maps), self).
+
+        type2:  x (ms,mt[,mu][,1]) -> k-space (M, nc) = forward(x * maps)
+        type1:  k-space (M, nc) -> image (ms,mt[,mu],1) = sum_c conj(maps_c) backward(y)_c
+        type21: image -> image = the composition with forward_backward."""
+        nc = int(maps.shape[-1])
+        dev = self._device_of(x, maps)
+        P = self._ensure(nc, dev)
+        fp = _fingerprint(maps)
+        if getattr(P, "_sense_key", None) != fp:
+            P.set_sense(maps)
+            P._sense_key = fp
+        img_shape = tuple(self.im_shape[:self.dim]) + (1,)
+        t = torch.as_tensor(x)
+        if kind == "type2":
+            out = P.sense_type2(t.reshape(P.modes))
+            return self._out(out, x, (P.M, nc))
+        if kind == "type1":
+            out = P.sense_type1(t.reshape(P.sample_shape()))
+            return self._out(out, x, img_shape)
+        out = P.sense_type21(t.reshape(P.modes))
+        return self._out(out, x, img_shape)
+
     @staticmethod
     def _out(out, like, shape):
         out = out.reshape(shape)
@@ -258,19 +282,47 @@ class None_opt:
 
 
 class joint2operators:
-    """B(A(x)) composition (pics/operators_class.py:496-514)."""
+    """B(A(x)) composition (pics/operators_class.py:496-514).
 
-    def __init__(self, Aopt, Bopt):
+    When A is ``espirit`` with coil-last maps of the operator's image shape and B a
+    NUFFT operator of this module, the three methods run the fused SENSE transforms
+    of the plan (the expansion x * S_c inside the first FFT pass, the combine
+    sum_c conj(S_c) inside the last one) instead of separate broadcasts; the results
+    equal the composition up to FP32 rounding order."""
+
+    def __init__(self, Aopt, Bopt, fused=True):
         self.Aopt = Aopt
         self.Bopt = Bopt
+        self.fused = fused
+
+    def _sense(self, xin):
+        """The NUFFT operator and the maps when the fused path applies, else None."""
+        A, B = self.Aopt, self.Bopt
+        if not (self.fused and isinstance(A, espirit) and isinstance(B, _NUFFTnd) and A.sens is not None):
+            return None
+        S = A.sens
+        d = B.dim
+        if B.im_shape is None or A.coil_axis != len(S.shape) - 1 or tuple(S.shape[:d]) != tuple(B.im_shape[:d]) \
+                or len(S.shape) != d + 1:
+            return None
+        return B
 
     def forward(self, xin):
+        B = self._sense(xin)
+        if B is not None:
+            return B._sense_apply("type2", self.Aopt.sens, xin)
         return self.Bopt.forward(self.Aopt.forward(xin))
 
     def backward(self, xin):
+        B = self._sense(xin)
+        if B is not None:
+            return B._sense_apply("type1", self.Aopt.sens, xin)
         return self.Aopt.backward(self.Bopt.backward(xin))
 
     def forward_backward(self, xin):
+        B = self._sense(xin)
+        if B is not None:
+            return B._sense_apply("type21", self.Aopt.sens, xin)
         if "forward_backward" in dir(self.Bopt):
             return self.Aopt.backward(self.Bopt.forward_backward(self.Aopt.forward(xin)))
         return self.Aopt.backward(self.Bopt.backward(self.Bopt.forward(self.Aopt.forward(xin))))

@@ -15,7 +15,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import GN_ADJOINT, GN_COIL_LAST, GN_COIL_MAJOR, GN_TYPE1, GN_TYPE2, GN_TYPE21, check
+from ._lib import (GN_ADJOINT, GN_COIL_LAST, GN_COIL_MAJOR, GN_SENSE_ADJOINT, GN_SENSE_TYPE1, GN_SENSE_TYPE2,
+                   GN_SENSE_TYPE21, GN_TYPE1, GN_TYPE2, GN_TYPE21, check)
 
 
 def grid_params(modes, eps, max_nspread=None):
@@ -182,6 +183,38 @@ class Plan:
     def type21(self, image):
         """type1(dcf * type2(image)): the normal operator of the reference."""
         return self._run(GN_TYPE21, image, self.image_shape(), self.image_shape())
+
+    # ------------------------------------------------------------------ SENSE
+    def set_sense(self, maps):
+        """Attach coil sensitivities (the plan's image layout with nc coils; copied into
+        the plan) for the sense_* transforms; None detaches them."""
+        if maps is None:
+            check(_lib.load().gn_plan_set_sense(self._h, None))
+            self._sense = None
+            return self
+        m = torch.as_tensor(maps).to(device=self.device, dtype=torch.complex64).reshape(self.image_shape())
+        m = m.contiguous()
+        with torch.cuda.device(self.device):
+            self._bind_stream()
+            check(_lib.load().gn_plan_set_sense(self._h, m.data_ptr()))
+        self._sense = m      # kept until the stream has copied it
+        return self
+
+    def sense_type2(self, image):
+        """type2(image * S): coil-less image (modes) -> samples of every coil."""
+        return self._run(GN_SENSE_TYPE2, image, self.modes, self.sample_shape())
+
+    def sense_type1(self, samples):
+        """sum_c conj(S_c) type1(samples)_c: samples -> coil-less image (modes)."""
+        return self._run(GN_SENSE_TYPE1, samples, self.sample_shape(), self.modes)
+
+    def sense_adjoint(self, samples):
+        """Exact adjoint of sense_type2."""
+        return self._run(GN_SENSE_ADJOINT, samples, self.sample_shape(), self.modes)
+
+    def sense_type21(self, image):
+        """sense_type1(dcf * sense_type2(image))."""
+        return self._run(GN_SENSE_TYPE21, image, self.modes, self.modes)
 
     def execute_into(self, kind, src, dst):
         """Run on preallocated contiguous complex64 device tensors (no allocation, no checks of shape)."""

@@ -20,31 +20,33 @@ from .plan import Plan
 
 
 def _bind(plan, coords):
-    """Make sure the plan holds exactly these coordinates."""
-    pts = getattr(plan, "_points", None)
-    same = pts is not None and len(pts) == coords.shape[0] and all(
-        torch.is_tensor(p) and p.data_ptr() == coords[i].data_ptr() and p._version == coords[i]._version
-        for i, p in enumerate(pts))
-    if not same:
+    """Make sure the plan holds exactly these coordinates.  The plan keeps a reference
+    to the tensor it binned (so its memory cannot be reused by another tensor while
+    bound) and the value of its version counter: another tensor, or an in-place
+    update of this one (or of a tensor it views), makes the plan re-bin."""
+    if getattr(plan, "_bound_coords", None) is not coords or plan._bound_ver != coords._version:
+        c32 = coords.detach().to(torch.float32).contiguous()
         # keep the plan's df and density weights (the exact adjoint ignores the weights)
-        plan.set_points([coords[i].detach() for i in range(coords.shape[0])], df=plan.df, dcf=plan._dcf)
+        plan.set_points([c32[i] for i in range(c32.shape[0])], df=plan.df, dcf=plan._dcf)
+        plan._bound_coords = coords
+        plan._bound_ver = coords._version
 
 
 class _Forward(torch.autograd.Function):
     @staticmethod
     def forward(ctx, image, coords, plan):
-        c32 = coords.detach().to(torch.float32).contiguous()
-        _bind(plan, c32)
+        _bind(plan, coords)
         y = plan.type2(image.detach().contiguous())
         ctx.plan = plan
-        ctx.save_for_backward(image, c32)
+        ctx.coords = coords
+        ctx.save_for_backward(image)
         return y
 
     @staticmethod
     def backward(ctx, gy):
-        image, c32 = ctx.saved_tensors
+        (image,) = ctx.saved_tensors
         plan = ctx.plan
-        _bind(plan, c32)
+        _bind(plan, ctx.coords)
         gy = gy.contiguous()
         g_img = plan.adjoint(gy) if ctx.needs_input_grad[0] else None
         g_crd = None
@@ -56,18 +58,18 @@ class _Forward(torch.autograd.Function):
 class _Adjoint(torch.autograd.Function):
     @staticmethod
     def forward(ctx, y, coords, plan):
-        c32 = coords.detach().to(torch.float32).contiguous()
-        _bind(plan, c32)
+        _bind(plan, coords)
         img = plan.adjoint(y.detach().contiguous())
         ctx.plan = plan
-        ctx.save_for_backward(y, c32)
+        ctx.coords = coords
+        ctx.save_for_backward(y)
         return img
 
     @staticmethod
     def backward(ctx, g_img):
-        y, c32 = ctx.saved_tensors
+        (y,) = ctx.saved_tensors
         plan = ctx.plan
-        _bind(plan, c32)
+        _bind(plan, ctx.coords)
         g_img = g_img.contiguous()
         g_y = plan.type2(g_img) if ctx.needs_input_grad[0] else None
         g_crd = None

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "gn_internal.h"
+#include "gn_host_plan.h"
 
 namespace gn {
 namespace {
@@ -517,31 +518,7 @@ __device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t lin
 // host side
 // ---------------------------------------------------------------------------
 
-std::vector<int> fft_radices(int L)
-{
-    std::vector<int> r;
-    int n = L;
-    while (n % 8 == 0 && n > 1) {
-        r.push_back(8);
-        n /= 8;
-    }
-    while (n % 4 == 0 && n > 1) {
-        r.push_back(4);
-        n /= 4;
-    }
-    while (n % 2 == 0 && n > 1) {
-        r.push_back(2);
-        n /= 2;
-    }
-    for (int p : {3, 5})
-        while (n % p == 0 && n > 1) {
-            r.push_back(p);
-            n /= p;
-        }
-    if (n != 1) return {};
-    std::reverse(r.begin(), r.end());
-    return r;
-}
+using host::fft_radices;
 
 __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_pad(RowArgs a) { fft_rows_pad_body(a, blockIdx.x, blockIdx.y); }
 __global__ __launch_bounds__(kFFTThreads) void k_fft_rows_crop(RowArgs a) { fft_rows_crop_body(a, blockIdx.x, blockIdx.y); }
@@ -749,23 +726,9 @@ int fft_axis_init(FFTAxis *ax, int L)
     ax->nst = (int)r.size();
     for (int i = 0; i < ax->nst; ++i) ax->radix[i] = r[i];
     // digit reversal: slot i holds position rev(i); pinv maps position -> slot
-    std::vector<int> pinv(L);
-    for (int i = 0; i < L; ++i) {
-        int rem = i, rev = 0, scale = L;
-        for (int rr : r) {
-            const int dgt = rem % rr;
-            rem /= rr;
-            scale /= rr;
-            rev += dgt * scale;
-        }
-        pinv[rev] = i;
-    }
-    std::vector<float2> tw(L);
-    const double two_pi = 6.283185307179586476925286766559;
-    for (int k = 0; k < L; ++k) {
-        const double ang = -two_pi * (double)k / (double)L;
-        tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-    }
+    std::vector<int> pinv;
+    std::vector<float> tw;
+    host::fft_tables(L, r, pinv, tw);
     if (hipMalloc((void **)&ax->tw, L * sizeof(float2)) != hipSuccess ||
         hipMalloc((void **)&ax->pinv, L * sizeof(int)) != hipSuccess)
         return fail(GN_ENOMEM, "FFT tables");

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gn_internal.h"
+#include "gn_host_plan.h"
 
 namespace gn {
 
@@ -1717,35 +1718,7 @@ int stable_sort_by_key(int64_t n, const int *keys, int stride, int64_t K, int *s
 
 void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gtile)
 {
-    if (dim == 3) {
-        tile[0] = 4;
-        tile[1] = 4;
-        tile[2] = 8;
-        gtile[0] = 10;
-        gtile[1] = 8;
-        gtile[2] = 12;
-    } else {
-        tile[0] = 8;
-        tile[1] = 16;
-        tile[2] = 1;
-        gtile[0] = 24;
-        gtile[1] = 24;
-        gtile[2] = 1;
-    }
-    // wide windows: shrink the gather tile (largest axis first) until tile + halo of a
-    // coil pair fits the 160 KiB of LDS
-    auto staged = [&]() {
-        int64_t r = kGatherCellBytes;
-        for (int a = 0; a < dim; ++a) r *= std::min<int64_t>(gtile[a] + 2 * ns - 1, nf[a]);
-        return r;
-    };
-    while (staged() > 160 * 1024) {
-        int a = 0;
-        for (int b = 1; b < dim; ++b)
-            if (gtile[b] > gtile[a]) a = b;
-        if (gtile[a] == 1) break;
-        --gtile[a];
-    }
+    host::default_tiles(dim, ns, nf, tile, gtile);
 }
 
 // Orders the records of one gather work item for conflict-free LDS reads in the
@@ -1932,56 +1905,28 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         GN_HIP(hipStreamSynchronize(st));
     }
 
-    // work items: every spread tile gets at least one item (empty tiles write zeros)
-    std::vector<int4> sitems, ritems, gitems;
-    int slots = 0;
-    for (int64_t k = 0; k < K; ++k) {
-        const int b = h_pair[k], e = h_pair[k + 1];
-        const int n = e - b;
-        if (n <= kSpreadChunk) {
-            sitems.push_back(make_int4((int)k, b, e, -1));
-        } else {
-            const int nch = (n + kSpreadChunk - 1) / kSpreadChunk;
-            ritems.push_back(make_int4((int)k, slots, nch, 0));
-            for (int c = 0; c < nch; ++c)
-                sitems.push_back(make_int4((int)k, b + c * kSpreadChunk, std::min(e, b + (c + 1) * kSpreadChunk),
-                                           slots + c));
-            slots += nch;
-        }
-    }
-    for (int64_t k = 0; k < Kg; ++k) {
-        const int hb = h_home[k], he = h_home[k + 1];
-        for (int s0 = hb; s0 < he; s0 += kGatherChunk)
-            gitems.push_back(make_int4((int)k, s0, std::min(he, s0 + kGatherChunk), 0));
-    }
-    if ((rc = dev_alloc(&R.spread_items, sitems.size(), "work items")) ||
-        (rc = dev_alloc(&R.reduce_items, std::max<size_t>(1, ritems.size()), "work items")) ||
-        (rc = dev_alloc(&R.gather_items, std::max<size_t>(1, gitems.size()), "work items")))
+    // work items: every spread tile gets at least one item (empty tiles write zeros),
+    // and the occupancy row masks for the first adjoint FFT pass (the spread leaves
+    // tiles without pairs unwritten where that pass skips them)
+    host::WorkItems wi;
+    host::build_work_items(K, Kg, g.ntiles, h_pair.data(), h_home.data(), kSpreadChunk, kGatherChunk, wi);
+    if ((rc = dev_alloc(&R.spread_items, wi.spread.size(), "work items")) ||
+        (rc = dev_alloc(&R.reduce_items, std::max<size_t>(1, wi.reduce.size()), "work items")) ||
+        (rc = dev_alloc(&R.gather_items, std::max<size_t>(1, wi.gather.size()), "work items")) ||
+        (rc = dev_alloc(&R.occ_mask, std::max<size_t>(1, wi.occ.size()), "tile masks")))
         return rc;
-    GN_HIP(hipMemcpy(R.spread_items, sitems.data(), sizeof(int4) * sitems.size(), hipMemcpyHostToDevice));
-    if (!ritems.empty())
-        GN_HIP(hipMemcpy(R.reduce_items, ritems.data(), sizeof(int4) * ritems.size(), hipMemcpyHostToDevice));
-    if (!gitems.empty())
-        GN_HIP(hipMemcpy(R.gather_items, gitems.data(), sizeof(int4) * gitems.size(), hipMemcpyHostToDevice));
-    R.n_spread_items = (int)sitems.size();
-    R.n_reduce_items = (int)ritems.size();
-    R.n_gather_items = (int)gitems.size();
-    R.n_slots = slots;
-    // Occupancy row masks for the first adjoint FFT pass (one word per column of spread
-    // tiles): the spread leaves tiles without pairs unwritten where the pass skips them.
-    {
-        const int64_t ncolt = (int64_t)g.ntiles[1] * g.ntiles[2]; // ntiles[2] = 1 in 2-D
-        const int kk = (g.ntiles[0] + 63) / 64;                   // leading-axis tiles per bit
-        std::vector<unsigned long long> occ((size_t)ncolt, 0ull);
-        for (const int4 &it : sitems)
-            if (it.z > it.y) {
-                const int64_t t0 = it.x / ncolt;
-                occ[(size_t)(it.x - t0 * ncolt)] |= 1ull << (t0 / kk);
-            }
-        if ((rc = dev_alloc(&R.occ_mask, (size_t)ncolt, "tile masks"))) return rc;
-        GN_HIP(hipMemcpy(R.occ_mask, occ.data(), sizeof(unsigned long long) * ncolt, hipMemcpyHostToDevice));
-        R.mask_k = kk;
-    }
+    GN_HIP(hipMemcpy(R.spread_items, wi.spread.data(), sizeof(int4) * wi.spread.size(), hipMemcpyHostToDevice));
+    if (!wi.reduce.empty())
+        GN_HIP(hipMemcpy(R.reduce_items, wi.reduce.data(), sizeof(int4) * wi.reduce.size(), hipMemcpyHostToDevice));
+    if (!wi.gather.empty())
+        GN_HIP(hipMemcpy(R.gather_items, wi.gather.data(), sizeof(int4) * wi.gather.size(), hipMemcpyHostToDevice));
+    GN_HIP(hipMemcpy(R.occ_mask, wi.occ.data(), sizeof(unsigned long long) * wi.occ.size(), hipMemcpyHostToDevice));
+    R.n_spread_items = (int)wi.spread.size();
+    R.n_reduce_items = (int)wi.reduce.size();
+    R.n_gather_items = (int)wi.gather.size();
+    R.n_slots = wi.slots;
+    R.mask_k = wi.mask_k;
+    const int slots = wi.slots;
     if (R.n_gather_items > 0) {
         const GatherArgs ga = make_gather_args(p, nullptr);
         if (g.dim == 3)

@@ -134,6 +134,7 @@ class Plan:
         self.M = M
         self._dcf = d
         self.df = float(df)
+        self._bound_coords = None     # the autograd functions' record of what is binned
         self._points = [c for c in coords]
         self._bound = (ts, f64, float(df), d)   # device copies, for re-binning by autotune
         return self

@@ -192,3 +192,36 @@ def test_bitwise_reproducible_across_plans():
         P = Plan((24, 24, 24), ncoils=2, rat=2, nspread=3).set_points(cs)
         outs.append(P.adjoint(y))
     assert torch.equal(outs[0], outs[1])
+
+
+def test_operator_sees_in_place_trajectory_and_dcf_updates():
+    """The operator re-bins when its trajectory or density weights change in place
+    (numpy arrays through a content hash, torch tensors through their version
+    counter), as the reference does by passing them to the driver on every call."""
+    import torch
+    from mripy_amd.operators import NUFFT2d_hip
+    print("case: operator in-place updates", flush=True)
+    rng = np.random.default_rng(9)
+    modes, M = (32, 28), 2000
+    x, y = [(np.pi * (2 * rng.random(M) - 1)).astype(np.float32) for _ in range(2)]
+    dcf = rng.random(M).astype(np.float32)
+    img = (rng.standard_normal(modes + (2,)) + 1j * rng.standard_normal(modes + (2,))).astype(np.complex64)
+    kd = (rng.standard_normal((M, 2)) + 1j * rng.standard_normal((M, 2))).astype(np.complex64)
+    op = NUFFT2d_hip(modes, dcf, x, y, rat=2, nspread=3)
+    op.forward(img)
+    op.ktrajx *= 0.5                     # numpy, in place
+    fresh = NUFFT2d_hip(modes, dcf, op.ktrajx.copy(), y, rat=2, nspread=3)
+    assert np.array_equal(op.forward(img), fresh.forward(img))
+    op.dcf[:] = 1.0                      # numpy weights, in place
+    fresh = NUFFT2d_hip(modes, np.ones(M, np.float32), op.ktrajx.copy(), y, rat=2, nspread=3)
+    assert np.array_equal(op.backward(kd), fresh.backward(kd))
+    # torch trajectory on the device, updated in place through a view
+    xt = torch.as_tensor(x).cuda()
+    yt = torch.as_tensor(y).cuda()
+    opt = NUFFT2d_hip(modes, None, xt, yt, rat=2, nspread=3)
+    it = torch.as_tensor(img).cuda()
+    a = opt.forward(it).clone()
+    xt.mul_(0.25)
+    b = opt.forward(it)
+    fresh = NUFFT2d_hip(modes, None, xt.clone(), yt, rat=2, nspread=3)
+    assert torch.equal(b, fresh.forward(it)) and not torch.equal(a, b)

@@ -1,0 +1,208 @@
+"""Bounds-checked CPU emulation of the set_points index math (no GPU needed).
+
+Restates, in NumPy, what the set_points kernels of mripy_amd/csrc/gn_kernels.hip
+compute for every sample -- k_prep (FP64 window start, offset, home gather tile),
+the digit-wise stable counting sort (k_radix_count / k_radix_scatter /
+k_bin_starts_sorted), k_pair_count / k_pair_emit (axis_tiles, tap_box: the spread
+tiles a window touches, the taps inside each, packed into 8- and 10-bit fields),
+k_bank_order (LDS-bank permutation of each gather work item) -- and the LDS
+addresses the gather and trajectory-gradient tap loops form from them, and asserts
+that every index stays inside the buffer the kernel writes or reads.  Geometries:
+the 3-D case (16, 20, 18) x 3 coils with 4000 samples on nf (32, 40, 36), the
+default-tile 3-D and 2-D cases, wide windows, and grids smaller than one tile.
+"""
+import numpy as np
+import pytest
+
+from oracle import nufft_oracle as O
+
+TWO_PI = 2 * np.pi
+KGATHER_CHUNK = 512
+KSPREAD_CHUNK = 512
+
+
+def default_tiles(dim, ns, nf):
+    """default_tiles + setup_tiles.fit (gn_kernels.hip, gn_api.hip)."""
+    tile, gtile = ([4, 4, 8], [10, 8, 12]) if dim == 3 else ([8, 16], [24, 24])
+
+    def staged():
+        r = 16
+        for a in range(dim):
+            r *= min(gtile[a] + 2 * ns - 1, nf[a])
+        return r
+    while staged() > 160 * 1024:
+        a = int(np.argmax(gtile[:dim]))
+        if gtile[a] == 1:
+            break
+        gtile[a] -= 1
+
+    def fit(t, a, tmax):
+        t = min(min(max(t, 1), tmax), nf[a])
+        if t + 2 * ns - 1 > nf[a]:
+            t = nf[a] if nf[a] <= tmax else nf[a] - 2 * ns + 1
+        return t
+    tile = [fit(tile[a], a, 32) for a in range(dim)]
+    gtile = [fit(gtile[a], a, 1 << 20) for a in range(dim)]
+    return tile, gtile
+
+
+def prep(coords, nf, ns, gtile):
+    """k_prep: window start s (wrapped), t = xi/h - m, home gather tile key."""
+    dim = len(nf)
+    ng = [-(-nf[a] // gtile[a]) for a in range(dim)]
+    s, t = [], []
+    key = np.zeros(coords[0].shape[0], np.int64)
+    for a in range(dim):
+        xi, m = O.cell_index(coords[a], nf[a])
+        h = TWO_PI / nf[a]
+        t.append(((xi - h * m) / h).astype(np.float32))
+        st = np.mod(m - ns, nf[a])
+        assert (st >= 0).all() and (st < nf[a]).all()
+        s.append(st)
+        key = key * ng[a] + st // gtile[a]
+    assert (key >= 0).all() and (key < np.prod(ng)).all()
+    return s, t, key, ng
+
+
+def axis_tiles(s, W, nf, T):
+    out, last = [], -1
+    for j in range(W):
+        c = s + j
+        if c >= nf:
+            c -= nf
+        q = c // T
+        if q != last:
+            out.append(q)
+            last = q
+    return out
+
+
+def tap_box(s, q, T, nf, W):
+    org = q * T
+    teff = min(T, nf - org)
+    if T == nf:
+        return 0, W, s
+    v = org - s
+    if v < 0:
+        v += nf
+    if v < W:
+        return v, min(W - v, teff), 0
+    return 0, min(W, teff - (s - org)), s - org
+
+
+def bank_order(cells, org, region, nf, dim):
+    """k_bank_order for one work item: position of each record, checked as a permutation."""
+    n = cells.shape[0]
+    assert n <= KGATHER_CHUNK
+    res = np.zeros(n, np.int64)
+    for i in range(n):
+        addr = 0
+        for a in range(dim):
+            la = cells[i, a] - org[a]
+            if la < 0:
+                la += nf[a]
+            assert 0 <= la < region[a]
+            addr = addr * region[a] + la if a else la
+        res[i] = addr & 15
+    rank = np.zeros(n, np.int64)
+    for v in range(16):
+        idx = np.nonzero(res == v)[0]
+        rank[idx] = np.arange(idx.size)
+    sz = np.bincount(res, minlength=16)
+    pos = np.array([sum(min(sz[v], k) + (1 if (v < r and sz[v] > k) else 0) for v in range(16))
+                    for k, r in zip(rank, res)])
+    assert sorted(pos.tolist()) == list(range(n))
+    return pos
+
+
+def emulate(coords, modes, ns, rat=2, check_pairs=True):
+    dim = len(modes)
+    nf = [max(rat * m, 2 * ns) for m in modes]
+    tile, gtile = default_tiles(dim, ns, nf)
+    W = 2 * ns
+    s, t, key, ng = prep(coords, nf, ns, gtile)
+    M = key.shape[0]
+    order = np.argsort(key, kind="stable")          # the counting sort's result (pinned on the GPU)
+    Kg = int(np.prod(ng))
+    bin_start = np.searchsorted(key[order], np.arange(Kg + 1), side="left")
+    cells = np.stack([s[a][order] for a in range(dim)], 1)
+    # gather work items and the LDS addresses of the tap loop
+    region = [min(gtile[a] + W - 1, nf[a]) for a in range(dim)]
+    assert 16 * np.prod(region) <= 160 * 1024
+    for k in range(Kg):
+        b, e = bin_start[k], bin_start[k + 1]
+        rr = k
+        org = [0] * dim
+        for a in reversed(range(dim)):
+            org[a] = (rr % ng[a]) * gtile[a]
+            rr //= ng[a]
+        for s0 in range(b, e, KGATHER_CHUNK):
+            c = cells[s0:min(e, s0 + KGATHER_CHUNK)]
+            bank_order(c, org, region, nf, dim)
+            for a in range(dim):
+                la = np.mod(c[:, a] - org[a], nf[a])
+                wrap = region[a] == nf[a]
+                hi = la + W - 1
+                if wrap:
+                    hi = np.where(hi >= region[a], hi - nf[a], hi)
+                assert (hi < region[a]).all() and (la >= 0).all()
+    if not check_pairs:
+        return
+    # spread pairs: tiles per axis, taps inside each, packed fields
+    ntiles = [-(-nf[a] // tile[a]) for a in range(dim)]
+    P = 0
+    keys = []
+    for i in range(M):
+        tl = [axis_tiles(int(cells[i, a]), W, nf[a], tile[a]) for a in range(dim)]
+        assert all(len(x) <= 32 for x in tl)
+        n = int(np.prod([len(x) for x in tl]))
+        for combo in np.ndindex(*[len(x) for x in tl]):
+            kk = 0
+            for a in range(dim):
+                q = tl[a][combo[a]]
+                j, cnt, l0 = tap_box(int(cells[i, a]), q, tile[a], nf[a], W)
+                assert 0 <= j < 256 and 1 <= cnt < 256 and 0 <= l0 < 1024
+                assert j + cnt <= W and l0 + cnt <= tile[a] if tile[a] != nf[a] else True
+                kk = kk * ntiles[a] + q
+            assert 0 <= kk < np.prod(ntiles)
+            keys.append(kk)
+        P += n
+    assert len(keys) == P and P < 2 ** 31
+    pstart = np.searchsorted(np.sort(np.array(keys)), np.arange(np.prod(ntiles) + 1))
+    assert pstart[-1] == P
+    # split-tile slots
+    sizes = np.diff(pstart)
+    assert (sizes >= 0).all()
+    return P
+
+
+def _random(M, dim, seed, spread=1.0):
+    rng = np.random.default_rng(seed)
+    return [(spread * np.pi * (2 * rng.random(M) - 1)).astype(np.float32).astype(np.float64) for _ in range(dim)]
+
+
+def test_traj_grad_geometry_16_20_18():
+    # the geometry of tests/test_gpu_fft.py::test_traj_grad_engines_agree (default_rng(3))
+    rng = np.random.default_rng(3)
+    cs = [(np.pi * (2 * rng.random(4000) - 1)).astype(np.float32).astype(np.float64) for _ in range(3)]
+    emulate(cs, (16, 20, 18), 3)
+
+
+@pytest.mark.parametrize("modes,ns,M", [((32, 32, 32), 3, 3000), ((24, 20, 18), 5, 2000), ((17, 19, 21), 3, 2000),
+                                        ((8, 8, 8), 6, 500), ((16, 16, 16), 8, 300)])
+def test_3d_geometries(modes, ns, M):
+    emulate(_random(M, 3, sum(modes) + ns), modes, ns)
+
+
+@pytest.mark.parametrize("modes,ns,rat,M", [((128, 128), 3, 2, 4000), ((40, 36), 6, 3, 3000), ((16, 16), 14, 3, 2000),
+                                            ((7, 9), 3, 2, 500)])
+def test_2d_geometries(modes, ns, rat, M):
+    emulate(_random(M, 2, sum(modes) + ns, spread=3.0), modes, ns, rat=rat)
+
+
+def test_wide_window_default_gather_tile_fits_lds():
+    for ns in (7, 8):
+        nf = [256, 256, 256]
+        tile, gtile = default_tiles(3, ns, nf)
+        staged = 16 * np.prod([min(g + 2 * ns - 1, n) for g, n in zip(gtile, nf)])
+        assert staged <= 160 * 1024, (ns, gtile)

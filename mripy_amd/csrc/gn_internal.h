@@ -104,6 +104,7 @@ struct Records {
     float4 *frac = nullptr;   // {t0, t1, t2, dcf}
     int *bin_start = nullptr; // Kg+1 offsets of each gather tile's samples
     int64_t cap = 0;
+    int64_t bin_start_cap = 0, spread_items_cap = 0, reduce_items_cap = 0, gather_items_cap = 0, occ_mask_cap = 0;
     // spread: one 32-byte record per (output tile, sample) pair, grouped by tile:
     //   a = {orig, j0 | j1<<8 | j2<<16, c0 | c1<<8 | c2<<16, l0 | l1<<10 | l2<<20}
     //   b = {t0, t1, t2, dcf} (float bits)

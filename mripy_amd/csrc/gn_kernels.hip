@@ -17,6 +17,9 @@
 #include <cstdio>
 #include <vector>
 
+#include <map>
+#include <mutex>
+
 #include "gn_internal.h"
 #include "gn_host_plan.h"
 
@@ -370,21 +373,19 @@ struct TileGeom {
     int W; // window taps per axis (2 nspread)
 };
 
-// distinct tiles touched by the window [s, s+W) mod nf along one axis, in tap order;
-// tiles are chosen so a window never re-enters a tile it left (T + W - 1 <= nf or T = nf)
-__device__ inline int axis_tiles(int s, int W, int nf, int T, int *out)
+// Tiles touched by the window [s, s+W) mod nf along one axis, in tap order: the
+// consecutive tiles q0, q0 + 1, ... (mod the tile count), n of them.  Tiles are
+// chosen so that a window never re-enters a tile it left (T + W - 1 <= nf or T = nf).
+__device__ inline void axis_tile_span(int s, int W, int nf, int T, int &q0, int &n)
 {
-    int n = 0, last = -1;
-    for (int j = 0; j < W; ++j) {
-        int c = s + j;
-        if (c >= nf) c -= nf;
-        const int t = c / T;
-        if (t != last) {
-            out[n++] = t;
-            last = t;
-        }
+    if (T == nf) {
+        q0 = 0;
+        n = 1;
+        return;
     }
-    return n;
+    q0 = s / T;
+    const int e = s + W - 1; // last cell, unwrapped
+    n = e < nf ? e / T - q0 + 1 : ((nf + T - 1) / T - q0) + (e - nf) / T + 1;
 }
 
 // window taps [jlo, jlo+cnt) of a window starting at s that fall in tile q, and the
@@ -419,9 +420,12 @@ __global__ __launch_bounds__(256) void k_pair_count(int64_t M, const int4 *__res
     if (p >= M) return;
     const int4 rc = rec_cell[p];
     const int s[3] = {rc.x, rc.y, rc.z};
-    int tl[32];
     int n = 1;
-    for (int ax = 0; ax < g.dim; ++ax) n *= axis_tiles(s[ax], g.W, (int)g.nf[ax], g.tile[ax], tl);
+    for (int ax = 0; ax < g.dim; ++ax) {
+        int q0, na;
+        axis_tile_span(s[ax], g.W, (int)g.nf[ax], g.tile[ax], q0, na);
+        n *= na;
+    }
     cnt[p] = n;
 }
 
@@ -434,34 +438,38 @@ __global__ __launch_bounds__(256) void k_pair_emit(int64_t M, const int4 *__rest
     if (p >= M) return;
     const int4 rc = rec_cell[p];
     const float4 rf = rec_frac[p];
-    int t0[32], t1[32], t2[32];
-    const int n0 = axis_tiles(rc.x, g.W, (int)g.nf[0], g.tile[0], t0);
-    const int n1 = axis_tiles(rc.y, g.W, (int)g.nf[1], g.tile[1], t1);
-    int n2 = 1;
-    t2[0] = 0;
-    if (g.dim == 3) n2 = axis_tiles(rc.z, g.W, (int)g.nf[2], g.tile[2], t2);
+    int q0[3] = {0, 0, 0}, n[3] = {1, 1, 1};
+    axis_tile_span(rc.x, g.W, (int)g.nf[0], g.tile[0], q0[0], n[0]);
+    axis_tile_span(rc.y, g.W, (int)g.nf[1], g.tile[1], q0[1], n[1]);
+    if (g.dim == 3) axis_tile_span(rc.z, g.W, (int)g.nf[2], g.tile[2], q0[2], n[2]);
     const int4 b = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
     int o = offs[p];
-    for (int i = 0; i < n0; ++i) {
+    int t0 = q0[0];
+    for (int i = 0; i < n[0]; ++i) {
         int j0, c0, l0;
-        tap_box(rc.x, t0[i], g.tile[0], (int)g.nf[0], g.W, j0, c0, l0);
-        for (int k = 0; k < n1; ++k) {
+        tap_box(rc.x, t0, g.tile[0], (int)g.nf[0], g.W, j0, c0, l0);
+        int t1 = q0[1];
+        for (int k = 0; k < n[1]; ++k) {
             int j1, c1, l1;
-            tap_box(rc.y, t1[k], g.tile[1], (int)g.nf[1], g.W, j1, c1, l1);
-            for (int m = 0; m < n2; ++m) {
+            tap_box(rc.y, t1, g.tile[1], (int)g.nf[1], g.W, j1, c1, l1);
+            int t2 = q0[2];
+            for (int m = 0; m < n[2]; ++m) {
                 int j2 = 0, c2 = 1, l2 = 0;
-                int key = t0[i] * g.ntiles[1] + t1[k];
+                int key = t0 * g.ntiles[1] + t1;
                 if (g.dim == 3) {
-                    tap_box(rc.z, t2[m], g.tile[2], (int)g.nf[2], g.W, j2, c2, l2);
-                    key = key * g.ntiles[2] + t2[m];
+                    tap_box(rc.z, t2, g.tile[2], (int)g.nf[2], g.W, j2, c2, l2);
+                    key = key * g.ntiles[2] + t2;
                 }
                 pkey[o] = key;
                 pa[o] = make_int4(rc.w, j0 | (j1 << 8) | (j2 << 16), c0 | (c1 << 8) | (c2 << 16),
                                   l0 | (l1 << 10) | (l2 << 20));
                 pb[o] = b;
                 ++o;
+                if (++t2 == g.ntiles[2]) t2 = 0;
             }
+            if (++t1 == g.ntiles[1]) t1 = 0;
         }
+        if (++t0 == g.ntiles[0]) t0 = 0;
     }
 }
 
@@ -1265,64 +1273,97 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int4 item
     const float2 *g0 = a.grid + (int64_t)cg0 * a.G;
     stage_region_pair<D>(lds_pair, g0, g0 + a.G, two, a.nf, org, R1, R2, RC);
     __syncthreads();
-    for (int j = item.y + threadIdx.x; j < item.z; j += kGatherThreads) {
-        const int4 rc = a.rec_cell[j];
-        const float4 rf = a.rec_frac[j];
-        int l0 = rc.x - org[0];
-        if (l0 < 0) l0 += (int)a.nf[0];
-        int l1 = rc.y - org[1];
-        if (l1 < 0) l1 += (int)a.nf[1];
-        int l2 = 0;
-        if (D == 3) {
-            l2 = rc.z - org[2];
-            if (l2 < 0) l2 += (int)a.nf[2];
-        }
-        float w1[W], w2[D == 3 ? W : 1];
+    // Threads per sample: a work item's samples are spread over all waves of the
+    // workgroup.  The tps lanes of a group share one sample and take the leading-axis
+    // taps t0 = sub, sub + tps, ... (tps divides W); lane sub 0 then adds the partial
+    // sums of lanes 1 .. tps-1 in that order, so the result does not depend on timing.
+    const int n = item.z - item.y;
+    const int nwaves = kGatherThreads / 64;
+    int tps = 1;
 #pragma unroll
-        for (int t = 0; t < W; ++t) {
-            const float u1 = rf.y + (float)(NS - t);
-            w1[t] = __builtin_amdgcn_exp2f(-a.coef[1] * u1 * u1);
-            if (D == 3) {
-                const float u2 = rf.z + (float)(NS - t);
-                w2[t] = __builtin_amdgcn_exp2f(-a.coef[2] * u2 * u2);
-            }
-        }
+    for (int d = 2; d <= 8; ++d)
+        if (W % d == 0 && n <= nwaves * (64 / d)) tps = d;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int spw = 64 / tps;               // samples per wave
+    const int grp = lane / tps, sub = lane - grp * tps;
+    for (int s0 = 0; s0 < n; s0 += nwaves * spw) {
+        const int slot = s0 + wave * spw + grp;
+        const bool valid = grp < spw && slot < n;
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll 1
-        for (int t0 = 0; t0 < W; ++t0) {
-            const float u0 = rf.x + (float)(NS - t0);
-            const float w0 = __builtin_amdgcn_exp2f(-a.coef[0] * u0 * u0);
-            int r0 = l0 + t0;
-            if (WRAP && r0 >= R0) r0 -= (int)a.nf[0]; // only when the region spans the whole axis
-            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        int orig = 0;
+        if (valid) {
+            const int j = item.y + slot;
+            const int4 rc = a.rec_cell[j];
+            const float4 rf = a.rec_frac[j];
+            orig = rc.w;
+            int l0 = rc.x - org[0];
+            if (l0 < 0) l0 += (int)a.nf[0];
+            int l1 = rc.y - org[1];
+            if (l1 < 0) l1 += (int)a.nf[1];
+            int l2 = 0;
+            if (D == 3) {
+                l2 = rc.z - org[2];
+                if (l2 < 0) l2 += (int)a.nf[2];
+            }
+            float w1[W], w2[D == 3 ? W : 1];
 #pragma unroll
-            for (int t1 = 0; t1 < W; ++t1) {
-                int r1 = l1 + t1;
-                if (WRAP && r1 >= R1) r1 -= (int)a.nf[1];
+            for (int t = 0; t < W; ++t) {
+                const float u1 = rf.y + (float)(NS - t);
+                w1[t] = __builtin_amdgcn_exp2f(-a.coef[1] * u1 * u1);
                 if (D == 3) {
-                    const int rowb = (r0 * R1 + r1) * R2;
-                    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                    for (int t2 = 0; t2 < W; ++t2) {
-                        int r2 = l2 + t2;
-                        if (WRAP && r2 >= R2) r2 -= (int)a.nf[2];
-                        fma4(q, w2[t2], lds_pair[rowb + r2]);
-                    }
-                    fma4(p, w1[t1], q);
-                } else {
-                    fma4(p, w1[t1], lds_pair[r0 * R1 + r1]);
+                    const float u2 = rf.z + (float)(NS - t);
+                    w2[t] = __builtin_amdgcn_exp2f(-a.coef[2] * u2 * u2);
                 }
             }
-            fma4(acc, w0, p);
+#pragma unroll 1
+            for (int t0 = sub; t0 < W; t0 += tps) {
+                const float u0 = rf.x + (float)(NS - t0);
+                const float w0 = __builtin_amdgcn_exp2f(-a.coef[0] * u0 * u0);
+                int r0 = l0 + t0;
+                if (WRAP && r0 >= R0) r0 -= (int)a.nf[0]; // only when the region spans the whole axis
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int t1 = 0; t1 < W; ++t1) {
+                    int r1 = l1 + t1;
+                    if (WRAP && r1 >= R1) r1 -= (int)a.nf[1];
+                    if (D == 3) {
+                        const int rowb = (r0 * R1 + r1) * R2;
+                        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                        for (int t2 = 0; t2 < W; ++t2) {
+                            int r2 = l2 + t2;
+                            if (WRAP && r2 >= R2) r2 -= (int)a.nf[2];
+                            fma4(q, w2[t2], lds_pair[rowb + r2]);
+                        }
+                        fma4(p, w1[t1], q);
+                    } else {
+                        fma4(p, w1[t1], lds_pair[r0 * R1 + r1]);
+                    }
+                }
+                fma4(acc, w0, p);
+            }
         }
+        // every lane takes part in the exchange (groups never straddle a wave)
+        for (int k = 1; k < tps; ++k) {
+            const int src = grp * tps + k;
+            const float4 o = make_float4(__shfl(acc.x, src), __shfl(acc.y, src), __shfl(acc.z, src),
+                                         __shfl(acc.w, src));
+            if (sub == 0) {
+                acc.x += o.x;
+                acc.y += o.y;
+                acc.z += o.z;
+                acc.w += o.w;
+            }
+        }
+        if (!valid || sub != 0) continue;
         const float2 o0 = make_float2(acc.x, acc.y), o1 = make_float2(acc.z, acc.w);
         if (a.layout == GN_COIL_LAST) {
-            float2 *o = a.out + (int64_t)rc.w * a.nc + cg0;
+            float2 *o = a.out + (int64_t)orig * a.nc + cg0;
             o[0] = o0;
             if (two) o[1] = o1;
         } else {
-            a.out[(int64_t)cg0 * a.M + rc.w] = o0;
-            if (two) a.out[(int64_t)(cg0 + 1) * a.M + rc.w] = o1;
+            a.out[(int64_t)cg0 * a.M + orig] = o0;
+            if (two) a.out[(int64_t)(cg0 + 1) * a.M + orig] = o1;
         }
     }
 }
@@ -1622,22 +1663,73 @@ int dev_alloc(T **p, size_t count, const char *what)
     return GN_OK;
 }
 
-// Scratch owner for one set_points call.
+// Device scratch of set_points: one pool per device, shared by every plan, that only
+// grows.  A set_points call holds the pool (its lock) until its last use of the
+// scratch has completed on its stream, so plans on other streams and threads never
+// see each other's buffers; buffers are requested in the same order on every call,
+// and buffer k of a call reuses buffer k of the previous one.
+struct ScratchPool {
+    std::mutex mu;
+    std::vector<void *> ptr;
+    std::vector<size_t> cap;
+};
+
+ScratchPool &scratch_pool()
+{
+    static std::mutex mu;
+    static std::map<int, ScratchPool *> pools;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> g(mu);
+    ScratchPool *&p = pools[dev];
+    if (!p) p = new ScratchPool(); // lives until the process ends
+    return *p;
+}
+
 struct Scratch {
-    std::vector<void *> ptrs;
+    ScratchPool &pool;
+    std::unique_lock<std::mutex> lock;
+    hipStream_t stream;
+    size_t next = 0;
+    explicit Scratch(hipStream_t st) : pool(scratch_pool()), lock(pool.mu), stream(st) {}
+    // also on early returns: nothing queued may still use the pool once it is released
+    ~Scratch() { (void)hipStreamSynchronize(stream); }
     template <typename T>
     int alloc(T **p, size_t count, const char *what)
     {
-        *p = nullptr;
-        int rc = dev_alloc(p, count, what);
-        if (!rc) ptrs.push_back((void *)*p);
-        return rc;
-    }
-    ~Scratch()
-    {
-        for (void *q : ptrs) (void)hipFree(q);
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        const size_t k = next++;
+        if (k >= pool.ptr.size()) {
+            pool.ptr.push_back(nullptr);
+            pool.cap.push_back(0);
+        }
+        if (pool.cap[k] < bytes) {
+            if (pool.ptr[k]) (void)hipFree(pool.ptr[k]); // waits for the device
+            pool.ptr[k] = nullptr;
+            pool.cap[k] = 0;
+            const size_t want = bytes + bytes / 4;      // headroom for slightly larger calls
+            if (hipMalloc(&pool.ptr[k], want) != hipSuccess) {
+                pool.ptr[k] = nullptr;
+                *p = nullptr;
+                return fail(GN_ENOMEM, std::string("hipMalloc failed for ") + what);
+            }
+            pool.cap[k] = want;
+        }
+        *p = (T *)pool.ptr[k];
+        return GN_OK;
     }
 };
+
+// A buffer owned by the plan that only grows (count elements of T, at least one).
+template <typename T>
+int grow_alloc(T **p, int64_t *cap, size_t count, const char *what)
+{
+    count = std::max<size_t>(count, 1);
+    if (*p && (size_t)*cap >= count) return GN_OK;
+    int rc = dev_alloc(p, count, what);
+    *cap = rc ? 0 : (int64_t)count;
+    return rc;
+}
 
 // Exclusive scan of n ints in place; writes the total to *total_dev (device int).
 hipError_t device_scan(int *data, int64_t n, int *bsum, hipStream_t st)
@@ -1814,10 +1906,10 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
             return rc;
         R.cap = M;
     }
-    if ((rc = dev_alloc(&R.bin_start, (size_t)Kg + 1, "bin offsets"))) return rc;
+    if ((rc = grow_alloc(&R.bin_start, &R.bin_start_cap, (size_t)Kg + 1, "bin offsets"))) return rc;
     R.npairs = 0;
 
-    Scratch sc;
+    Scratch sc(st);
     std::vector<int> h_home(Kg + 1, 0), h_pair(K + 1, 0);
 
     if (M > 0) {
@@ -1910,10 +2002,10 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
     // tiles without pairs unwritten where that pass skips them)
     host::WorkItems wi;
     host::build_work_items(K, Kg, g.ntiles, h_pair.data(), h_home.data(), kSpreadChunk, kGatherChunk, wi);
-    if ((rc = dev_alloc(&R.spread_items, wi.spread.size(), "work items")) ||
-        (rc = dev_alloc(&R.reduce_items, std::max<size_t>(1, wi.reduce.size()), "work items")) ||
-        (rc = dev_alloc(&R.gather_items, std::max<size_t>(1, wi.gather.size()), "work items")) ||
-        (rc = dev_alloc(&R.occ_mask, std::max<size_t>(1, wi.occ.size()), "tile masks")))
+    if ((rc = grow_alloc(&R.spread_items, &R.spread_items_cap, wi.spread.size(), "work items")) ||
+        (rc = grow_alloc(&R.reduce_items, &R.reduce_items_cap, wi.reduce.size(), "work items")) ||
+        (rc = grow_alloc(&R.gather_items, &R.gather_items_cap, wi.gather.size(), "work items")) ||
+        (rc = grow_alloc(&R.occ_mask, &R.occ_mask_cap, wi.occ.size(), "tile masks")))
         return rc;
     GN_HIP(hipMemcpy(R.spread_items, wi.spread.data(), sizeof(int4) * wi.spread.size(), hipMemcpyHostToDevice));
     if (!wi.reduce.empty())

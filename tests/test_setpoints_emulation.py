@@ -77,6 +77,16 @@ def axis_tiles(s, W, nf, T):
     return out
 
 
+def axis_tile_span(s, W, nf, T):
+    """The kernels' closed form of axis_tiles: first tile and count (consecutive mod ntiles)."""
+    if T == nf:
+        return 0, 1
+    q0 = s // T
+    e = s + W - 1
+    n = e // T - q0 + 1 if e < nf else (-(-nf // T) - q0) + (e - nf) // T + 1
+    return q0, n
+
+
 def tap_box(s, q, T, nf, W):
     org = q * T
     teff = min(T, nf - org)
@@ -154,7 +164,9 @@ def emulate(coords, modes, ns, rat=2, check_pairs=True):
     keys = []
     for i in range(M):
         tl = [axis_tiles(int(cells[i, a]), W, nf[a], tile[a]) for a in range(dim)]
-        assert all(len(x) <= 32 for x in tl)
+        for a in range(dim):
+            q0, na = axis_tile_span(int(cells[i, a]), W, nf[a], tile[a])
+            assert tl[a] == [(q0 + k) % ntiles[a] for k in range(na)]
         n = int(np.prod([len(x) for x in tl]))
         for combo in np.ndindex(*[len(x) for x in tl]):
             kk = 0

@@ -166,6 +166,26 @@ int gn_plan_set_fft_engine(gn_plan *plan, int engine);
 
 int gn_plan_destroy(gn_plan *plan);
 
+/* ---- Cartesian FFTs -------------------------------------------------------------
+ * Batched complex-to-complex FFT over the leading `rank` (1..3) axes of a C-order
+ * complex64 array of shape (shape[0], .., shape[rank-1], batch) whose batch elements
+ * (coils) are interleaved: element (i, b) at i * batch + b.  Replaces the cuFFT
+ * wrappers fftnc2c_cuda / ifftnc2c_cuda / fft2c2c_cuda / ifft2c2c_cuda
+ * (fft/cufft.py:27-97: forward unnormalized, inverse divided by prod(shape)) and, with
+ * the shift flags and the mask, the transforms of FFTnd_cuda_kmask / FFT2d_cuda_kmask
+ * (pics/operators_cuda_class.py:7-58: forward = ifftshift(FFT(fftshift(im))) * mask,
+ * backward = ifftshift(IFFT(fftshift(ksp))), shifts over the transformed axes).
+ * mask: null or float32 of prod(shape) entries (broadcast over the batch, mask_batch
+ * = 1) or prod(shape) * batch entries (mask_batch = batch); it multiplies the output.
+ * in and out may be the same buffer.  Host-synchronous on `stream` (the shifted copy
+ * uses one workspace per device): calls from several threads are serialized. */
+#define GN_FFT_FORWARD 1
+#define GN_FFT_INVERSE -1
+#define GN_FFT_SHIFT_IN 1  /* fftshift the input over the transformed axes  */
+#define GN_FFT_SHIFT_OUT 2 /* ifftshift the output over the transformed axes */
+int gn_fft_c2c(int rank, const int64_t *shape, int64_t batch, int direction, int flags, const float *mask,
+               int64_t mask_batch, const void *in, void *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgaussnufft.so")
-SOURCES = ["gn_api.hip", "gn_kernels.hip", "gn_solver.hip", "gn_batch.hip", "gn_fft.hip"]
+SOURCES = ["gn_api.hip", "gn_kernels.hip", "gn_solver.hip", "gn_batch.hip", "gn_fft.hip", "gn_cfft.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIPCC_FLAGS = [

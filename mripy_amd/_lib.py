@@ -20,6 +20,7 @@ GN_COIL_LAST, GN_COIL_MAJOR = 0, 1
 GN_TYPE1, GN_TYPE2, GN_ADJOINT, GN_TYPE21 = 1, 2, 3, 21
 GN_SENSE_TYPE2, GN_SENSE_TYPE1, GN_SENSE_ADJOINT, GN_SENSE_TYPE21 = 32, 31, 33, 321
 GN_FFT_AUTO, GN_FFT_ROCFFT = 0, 1
+GN_FFT_FORWARD, GN_FFT_INVERSE, GN_FFT_SHIFT_IN, GN_FFT_SHIFT_OUT = 1, -1, 1, 2
 STAGES = ("pad", "fft2", "gather", "spread", "fft1", "crop")
 GN_F32, GN_C64, GN_C128 = 1, 2, 3
 GN_TV_AMP_SQUARE, GN_TV_AMP_MODULUS = 0, 1
@@ -39,6 +40,8 @@ SIGNATURES = {
     "gn_set_points": ([_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
     "gn_execute": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "gn_plan_set_sense": ([_vp, _vp], ctypes.c_int),
+    "gn_fft_c2c": ([ctypes.c_int, _i64p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp, _vp,
+                    _vp], ctypes.c_int),
     "gn_plan_info": ([_vp, _i64p, _ip, ctypes.POINTER(ctypes.c_double), _i64p, _i64p, _ip], ctypes.c_int),
     "gn_plan_set_tiles": ([_vp, _ip], ctypes.c_int),
     "gn_plan_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),

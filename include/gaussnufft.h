@@ -62,6 +62,11 @@ extern "C" {
 #define GN_SENSE_TYPE1 31    /* x = sum_c conj(S_c) GN_TYPE1(y)_c (with dcf)                  */
 #define GN_SENSE_ADJOINT 33  /* exact adjoint of GN_SENSE_TYPE2 (no dcf, no 1/M)              */
 #define GN_SENSE_TYPE21 321  /* GN_SENSE_TYPE1(GN_SENSE_TYPE2(x)) with dcf                    */
+/* The gridding step alone (the reference's build_grid_{2d,3d}{1,2}_fast,
+ * fft/nufft_func.py:550-639, 863-1055), on a caller grid laid out coil-major
+ * (nc, nf[0], .., nf[d-1]) complex64:                                                  */
+#define GN_GRID_SPREAD 41    /* grid[g] = sum_j w_j(g) y_j (no dcf, no scale)               */
+#define GN_GRID_GATHER 42    /* y_j = (1/G) sum_g w_j(g) grid[g]                            */
 
 /* FFT engines for gn_plan_set_fft_engine */
 #define GN_FFT_AUTO 0   /* pruned in-place passes fused with pad/crop when every nf is a product of

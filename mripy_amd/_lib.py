@@ -19,6 +19,7 @@ GN_OK, GN_EINVAL, GN_ENOMEM, GN_EHIP, GN_EROCFFT, GN_ESTATE = 0, 1, 2, 3, 4, 5
 GN_COIL_LAST, GN_COIL_MAJOR = 0, 1
 GN_TYPE1, GN_TYPE2, GN_ADJOINT, GN_TYPE21 = 1, 2, 3, 21
 GN_SENSE_TYPE2, GN_SENSE_TYPE1, GN_SENSE_ADJOINT, GN_SENSE_TYPE21 = 32, 31, 33, 321
+GN_GRID_SPREAD, GN_GRID_GATHER = 41, 42
 GN_FFT_AUTO, GN_FFT_ROCFFT = 0, 1
 GN_FFT_FORWARD, GN_FFT_INVERSE, GN_FFT_SHIFT_IN, GN_FFT_SHIFT_OUT = 1, -1, 1, 2
 STAGES = ("pad", "fft2", "gather", "spread", "fft1", "crop")

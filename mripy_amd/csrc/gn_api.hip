@@ -403,6 +403,19 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
     if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called before gn_execute");
     const bool sense = kind == GN_SENSE_TYPE1 || kind == GN_SENSE_TYPE2 || kind == GN_SENSE_ADJOINT ||
                        kind == GN_SENSE_TYPE21;
+    if (kind == GN_GRID_SPREAD || kind == GN_GRID_GATHER) {
+        // the gridding step alone on a caller grid (coil-major nc x nf[0] x .. x nf[d-1])
+        if (!in || !out) return fail(GN_EINVAL, "null data pointer");
+        if (p->M == 0) {
+            if (kind == GN_GRID_SPREAD) {
+                hipError_t e = hipMemsetAsync(out, 0, (size_t)(p->geo.G * p->nc) * sizeof(float2), p->stream);
+                if (e != hipSuccess) return hip_fail(e, "grid clear");
+            }
+            return GN_OK;
+        }
+        if (kind == GN_GRID_SPREAD) return launch_spread(p, (const float2 *)in, 0, (float2 *)out);
+        return launch_gather(p, (float2 *)out, (const float2 *)in, (float)(1.0 / (double)p->geo.G));
+    }
     if (kind != GN_TYPE1 && kind != GN_TYPE2 && kind != GN_ADJOINT && kind != GN_TYPE21 && !sense)
         return fail(GN_EINVAL, "unknown transform kind");
     if (sense && !p->sens) return fail(GN_ESTATE, "gn_plan_set_sense must be called before a SENSE transform");

@@ -193,6 +193,7 @@ struct GatherArgs {
     const float4 *rec_frac;
     const float2 *grid;
     float2 *out;
+    float scale = 1.0f; // output factor (1 in the transforms; 1/G for the raw gridding kind)
 };
 
 struct PrepostArgs {
@@ -271,8 +272,9 @@ int hip_fail(hipError_t e, const char *what);
 // kernels.hip
 int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const void *z, int coord_f64, double df,
                  const float *dcf);
-int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf);
-int launch_gather(gn_plan_s *p, float2 *samples);
+// grid: null = the plan's grid (the transforms); else a caller grid, every tile written
+int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf, float2 *grid = nullptr);
+int launch_gather(gn_plan_s *p, float2 *samples, const float2 *grid = nullptr, float scale = 1.0f);
 int launch_pad(gn_plan_s *p, const float2 *image, float scale, bool sense = false);
 int launch_crop(gn_plan_s *p, float2 *image, float scale, bool sense = false);
 int launch_traj_grad(gn_plan_s *p, const float2 *wts, float *out);

@@ -1356,6 +1356,12 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int4 item
             }
         }
         if (!valid || sub != 0) continue;
+        if (a.scale != 1.0f) {
+            acc.x *= a.scale;
+            acc.y *= a.scale;
+            acc.z *= a.scale;
+            acc.w *= a.scale;
+        }
         const float2 o0 = make_float2(acc.x, acc.y), o1 = make_float2(acc.z, acc.w);
         if (a.layout == GN_COIL_LAST) {
             float2 *o = a.out + (int64_t)orig * a.nc + cg0;
@@ -2237,11 +2243,13 @@ SpreadArgs make_spread_args(const gn_plan_s *p, const float2 *samples, int use_d
     return a;
 }
 
-int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf)
+int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf, float2 *grid)
 {
     const Geometry &g = p->geo;
     SpreadArgs a = make_spread_args(p, samples, use_dcf);
-    if (p->fft_custom && p->fft_engine == GN_FFT_AUTO) {
+    if (grid) {
+        a.grid = grid; // every tile written (no pass skips cells of this grid)
+    } else if (p->fft_custom && p->fft_engine == GN_FFT_AUTO) {
         // the pruned pass C' reads only occupied tiles: empty ones need no zeros
         a.occ = p->rec.occ_mask;
         a.occ_k = p->rec.mask_k;
@@ -2275,11 +2283,13 @@ GatherArgs make_gather_args(const gn_plan_s *p, float2 *samples)
     return a;
 }
 
-int launch_gather(gn_plan_s *p, float2 *samples)
+int launch_gather(gn_plan_s *p, float2 *samples, const float2 *grid, float scale)
 {
     const Geometry &g = p->geo;
     if (p->M == 0 || p->rec.n_gather_items == 0) return GN_OK;
-    const GatherArgs a = make_gather_args(p, samples);
+    GatherArgs a = make_gather_args(p, samples);
+    if (grid) a.grid = grid;
+    a.scale = scale;
     hipError_t e = gather_any(g.dim, g.ns, a, p->rec.n_gather_items, p->nc, p->stream);
     if (e != hipSuccess) return hip_fail(e, "gather launch");
     return GN_OK;

@@ -15,8 +15,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (GN_ADJOINT, GN_COIL_LAST, GN_COIL_MAJOR, GN_SENSE_ADJOINT, GN_SENSE_TYPE1, GN_SENSE_TYPE2,
-                   GN_SENSE_TYPE21, GN_TYPE1, GN_TYPE2, GN_TYPE21, check)
+from ._lib import (GN_ADJOINT, GN_COIL_LAST, GN_COIL_MAJOR, GN_GRID_GATHER, GN_GRID_SPREAD, GN_SENSE_ADJOINT,
+                   GN_SENSE_TYPE1, GN_SENSE_TYPE2, GN_SENSE_TYPE21, GN_TYPE1, GN_TYPE2, GN_TYPE21, check)
 
 
 def grid_params(modes, eps, max_nspread=None):
@@ -184,6 +184,17 @@ class Plan:
     def type21(self, image):
         """type1(dcf * type2(image)): the normal operator of the reference."""
         return self._run(GN_TYPE21, image, self.image_shape(), self.image_shape())
+
+    # ------------------------------------------------------------------ gridding alone
+    def grid_spread(self, samples):
+        """The gridding step of type 1 (build_grid_*1_fast, fft/nufft_func.py:550-582,
+        863-909): samples -> oversampled grid (nc, *nf), no dcf, no scale."""
+        return self._run(GN_GRID_SPREAD, samples, self.sample_shape(), (self.ncoils,) + self.nf)
+
+    def grid_gather(self, grid):
+        """The gridding step of type 2 (build_grid_*2_fast, fft/nufft_func.py:607-639,
+        985-1032): oversampled grid (nc, *nf) -> samples, divided by G."""
+        return self._run(GN_GRID_GATHER, grid, (self.ncoils,) + self.nf, self.sample_shape())
 
     # ------------------------------------------------------------------ SENSE
     def set_sense(self, maps):

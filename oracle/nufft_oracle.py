@@ -373,6 +373,20 @@ def traj_grad_on_grid(coords, grid, w, p: Params):
     return out
 
 
+def pipe_menon(coords, modes, niter=10, rat=2, nspread=3):
+    """FP64 Pipe-Menon density weights on the gridding kernel (no reference
+    counterpart): w <- w / |gather(spread(w))|, normalised to a maximum of 1.
+    Pinned by the Cartesian-lattice case (uniform weights) in tests/test_oracle.py."""
+    p = Params(modes, rat=rat, nspread=nspread)
+    coords = [np.asarray(x, dtype=np.float64) for x in coords]
+    w = np.ones(coords[0].shape[0], dtype=np.complex128)
+    for _ in range(int(niter)):
+        c = gather(coords, spread(coords, w, p.nf, p.ns, p.tau), p.nf, p.ns, p.tau)[:, 0]
+        w = w / np.abs(c)
+    w = w.real
+    return w / w.max()
+
+
 # ----------------------------------------------------------------------------
 # reference function names (fft/nufft_func.py:702,741,780,1167,1209,1261)
 # ----------------------------------------------------------------------------

@@ -250,3 +250,23 @@ def test_cell_index_python_semantics():
         assert b == 1 + int(r // h)
     mc, _ = C.cell_index(xs, nf)
     assert (mc == m).all()
+
+
+def test_pipe_menon_uniform_on_a_lattice_and_radial_ramp():
+    # samples on a Cartesian lattice (a quarter period off the grid cells, so that every
+    # window covers the same taps) have equal density: every weight is 1
+    m = 16
+    n = np.arange(m) - m // 2 + 0.25
+    X, Y = np.meshgrid(2 * np.pi * n / m, 2 * np.pi * n / m, indexing="ij")
+    w = O.pipe_menon([X.ravel(), Y.ravel()], (m, m), niter=5)
+    assert np.abs(w - 1).max() < 1e-12
+    # a radial trajectory is dense at the centre: weights grow with |k| away from it
+    th = np.arange(64) * np.pi / 64
+    r = (np.arange(64) - 32) / 64
+    x = (2 * np.pi * r[None, :] * np.cos(th)[:, None]).ravel()
+    y = (2 * np.pi * r[None, :] * np.sin(th)[:, None]).ravel()
+    w = O.pipe_menon([x, y], (32, 32), niter=10)
+    rad = np.hypot(x, y)
+    inner = w[(rad > 0.3) & (rad < 0.6)].mean()
+    outer = w[(rad > 1.8) & (rad < 2.4)].mean()
+    assert outer > 2.5 * inner

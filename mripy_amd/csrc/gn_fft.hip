@@ -304,141 +304,6 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
     }
 }
 
-// Persistent column pass: a block walks units u = blockIdx.x, + gridDim.x, ... (unit =
-// column block + slab * ncb) and keeps the next unit's input loads in flight while it
-// transforms and stores the current one.  Up to kPipeLoads 16-byte loads per thread
-// are held in registers; the host uses this form only when a unit's input fits.
-constexpr int kPipeLoads = 4;
-
-// two columns per lane (16-byte accesses): every unit's row segments are 16-byte
-// aligned (the same answer for every unit, host and device)
-__host__ __device__ inline bool cols_v2(const ColArgs &a)
-{
-    return (((a.ncols | a.stride_outer | a.stride_inner | (int64_t)a.CB) & 1) == 0) &&
-           ((reinterpret_cast<uintptr_t>(a.data) & 15) == 0);
-}
-
-template <int MASK>
-__device__ __forceinline__ void cols_fetch(const ColArgs &a, int64_t u, int64_t ncb, bool v2, int ws, int cbws,
-                                           int cbw, float4 *v)
-{
-    const int64_t slab = u / ncb, colblk = u - slab * ncb;
-    const int64_t col0 = colblk * a.CB;
-    const int ncb_cols = (int)min((int64_t)a.CB, a.ncols - col0);
-    const uint32_t so = (uint32_t)slab / (uint32_t)a.slab_inner, si = (uint32_t)slab - so * (uint32_t)a.slab_inner;
-    const float2 *base = a.data + so * a.stride_outer + si * a.stride_inner + col0;
-    const int total_in = a.rows_in * cbw;
-    unsigned long long mk0 = ~0ull, mk1 = ~0ull;
-    if (MASK) {
-        const int j = (threadIdx.x & (cbw - 1)) << ws;
-        if (j < ncb_cols) mk0 = col_mask(a, a.rmask, col0 + j);
-        mk1 = (v2 && j + 1 < ncb_cols) ? col_mask(a, a.rmask, col0 + j + 1) : mk0;
-    }
-    const float rmrow = 1.0f / (float)a.mrow;
-#pragma unroll
-    for (int k = 0; k < kPipeLoads; ++k) {
-        const int e = threadIdx.x + k * blockDim.x;
-        const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
-        v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (e < total_in && j < ncb_cols) {
-            bool on0 = true, on1 = true;
-            if (MASK == 1) {
-                const int b = fdiv(n, a.mrow, rmrow);
-                on0 = (mk0 >> b) & 1ull;
-                on1 = (mk1 >> b) & 1ull;
-            }
-            if (on0 || on1) {
-                const float2 *src = base + (int64_t)n * a.ncols + j;
-                if (v2) {
-                    v[k] = *reinterpret_cast<const float4 *>(src);
-                    if (!on0) v[k].x = v[k].y = 0.f;
-                    if (!on1) v[k].z = v[k].w = 0.f;
-                } else {
-                    const float2 t = *src;
-                    v[k] = make_float4(t.x, t.y, 0.f, 0.f);
-                }
-            }
-        }
-    }
-}
-
-template <int MASK>
-__device__ __forceinline__ void fft_cols_persist_body(const ColArgs &a)
-{
-    extern __shared__ float2 lds_fft[];
-    float2 *tw = lds_fft;
-    float2 *buf = lds_fft + a.L;
-    const int CB = a.CB;
-    const int estride = CB + 1;
-    const int64_t ncb = (a.ncols + CB - 1) / CB;
-    const int64_t nunits = ncb * a.nslabs;
-    const int cbs = __builtin_ctz(CB);
-    const bool v2 = cols_v2(a);
-    const int ws = v2 ? 1 : 0;
-    const int cbw = CB >> ws, cbws = cbs - ws;
-    const int total_in = a.rows_in * cbw;
-    const int half_in = a.rows_in / 2, half_out = a.rows_out / 2;
-    int *spinv = (int *)(buf + a.L * estride);
-    load_table(tw, a.tw, a.L, a.sign);
-    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
-    float4 v[kPipeLoads];
-    int64_t u = blockIdx.x;
-    if (u < nunits) cols_fetch<MASK>(a, u, ncb, v2, ws, cbws, cbw, v);
-    for (; u < nunits; u += gridDim.x) {
-        if (a.rows_in < a.L) // pruned input: zero every slot first
-            for (int e = threadIdx.x; e < a.L * estride; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
-        __syncthreads(); // tables staged, buf zeroed
-#pragma unroll
-        for (int k = 0; k < kPipeLoads; ++k) {
-            const int e = threadIdx.x + k * blockDim.x;
-            const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
-            if (e < total_in) {
-                int p = n;
-                if (a.in_map) {
-                    p = n - half_in;
-                    if (p < 0) p += a.L;
-                }
-                float2 *dst = buf + spinv[p] * estride + j;
-                dst[0] = make_float2(v[k].x, v[k].y);
-                if (v2) dst[1] = make_float2(v[k].z, v[k].w);
-            }
-        }
-        const int64_t un = u + gridDim.x;
-        if (un < nunits) cols_fetch<MASK>(a, un, ncb, v2, ws, cbws, cbw, v); // in flight during the FFT
-        __syncthreads();
-        fft_lines(buf, CB, 1, estride, a.L, a.st, tw, a.sign);
-        const int64_t slab = u / ncb, colblk = u - slab * ncb;
-        const int64_t col0 = colblk * CB;
-        const int ncb_cols = (int)min((int64_t)CB, a.ncols - col0);
-        const uint32_t so = (uint32_t)slab / (uint32_t)a.slab_inner, si = (uint32_t)slab - so * (uint32_t)a.slab_inner;
-        float2 *base = a.data + so * a.stride_outer + si * a.stride_inner + col0;
-        for (int e = threadIdx.x; e < a.rows_out * cbw; e += blockDim.x) {
-            const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
-            if (j >= ncb_cols) continue;
-            int p = n;
-            if (a.out_map) {
-                p = n - half_out;
-                if (p < 0) p += a.L;
-            }
-            const float2 *src = buf + p * estride + j;
-            float2 *dst = base + (int64_t)n * a.ncols + j;
-            if (v2) {
-                const float2 lo = src[0], hi = src[1];
-                *reinterpret_cast<float4 *>(dst) = make_float4(lo.x, lo.y, hi.x, hi.y);
-            } else {
-                *dst = src[0];
-            }
-        }
-        __syncthreads(); // every store has read buf before the next unit overwrites it
-    }
-}
-
-template <int MASK>
-__global__ __launch_bounds__(kFFTColThreads) void k_fft_cols_persist(ColArgs a)
-{
-    fft_cols_persist_body<MASK>(a);
-}
-
 template <int MASK>
 __global__ __launch_bounds__(kFFTColThreads) void k_fft_cols(ColArgs a)
 {
@@ -729,31 +594,10 @@ void allow_lds(const void *kernel, size_t lds)
     if (lds > 64 * 1024) (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-#ifndef GN_FFT_PERSIST
-#define GN_FFT_PERSIST 1
-#endif
-
 hipError_t launch_cols(const ColArgs &a, hipStream_t st)
 {
     dim3 grid((unsigned)((a.ncols + a.CB - 1) / a.CB), (unsigned)a.nslabs);
     const size_t lds = col_lds(a.L, a.CB);
-    // persistent, pipelined form: a unit's input must fit kPipeLoads loads per thread
-    const int cbw = cols_v2(a) ? a.CB / 2 : a.CB;
-    if (GN_FFT_PERSIST && (int64_t)a.rows_in * cbw <= (int64_t)kPipeLoads * kFFTColThreads) {
-        const int64_t units = (int64_t)grid.x * grid.y;
-        int dev = 0, ncu = 256;
-        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
-        const int64_t nb = std::min<int64_t>(units, (int64_t)ncu * std::min(per_cu, 4));
-        if (a.rmask) {
-            allow_lds((const void *)k_fft_cols_persist<1>, lds);
-            hipLaunchKernelGGL(k_fft_cols_persist<1>, dim3((unsigned)nb), dim3(kFFTColThreads), lds, st, a);
-        } else {
-            allow_lds((const void *)k_fft_cols_persist<0>, lds);
-            hipLaunchKernelGGL(k_fft_cols_persist<0>, dim3((unsigned)nb), dim3(kFFTColThreads), lds, st, a);
-        }
-        return hipGetLastError();
-    }
     if (a.rmask) {
         allow_lds((const void *)k_fft_cols<1>, lds);
         hipLaunchKernelGGL(k_fft_cols<1>, grid, dim3(kFFTColThreads), lds, st, a);

@@ -1420,17 +1420,29 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
     }
     const int R0 = a.region[0], R1 = a.region[1], R2 = (D == 3) ? a.region[2] : 1;
     const int RC = R0 * R1 * R2;
-    const int R12 = R1 * R2;
-    for (int jb = item.y; jb < item.z; jb += kGatherThreads) {
-        const int j = jb + threadIdx.x;
-        const bool valid = j < item.z;
+    // several lanes per sample, as in the gather: the tps lanes of a group take the
+    // leading-axis taps t0 = sub, sub + tps, ... and the group's lane 0 adds the
+    // partial gradients of the others in lane order
+    const int n = item.z - item.y;
+    const int nwaves = kGatherThreads / 64;
+    int tps = 1;
+#pragma unroll
+    for (int d = 2; d <= 8; ++d)
+        if (W % d == 0 && n <= nwaves * (64 / d)) tps = d;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int spw = 64 / tps;
+    const int grp = lane / tps, sub = lane - grp * tps;
+    for (int s0 = 0; s0 < n; s0 += nwaves * spw) {
+        const int slot = s0 + wave * spw + grp;
+        const bool valid = grp < spw && slot < n;
         float g0 = 0.f, g1 = 0.f, g2 = 0.f;
         int4 rc = make_int4(0, 0, 0, 0);
         float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
         int l0 = 0, l1 = 0, l2 = 0;
+        float w1[W], u1s[W], w2[D == 3 ? W : 1], u2s[D == 3 ? W : 1];
         if (valid) {
-            rc = a.rec_cell[j];
-            rf = a.rec_frac[j];
+            rc = a.rec_cell[item.y + slot];
+            rf = a.rec_frac[item.y + slot];
             l0 = rc.x - org[0];
             if (l0 < 0) l0 += (int)a.nf[0];
             l1 = rc.y - org[1];
@@ -1438,6 +1450,15 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
             if (D == 3) {
                 l2 = rc.z - org[2];
                 if (l2 < 0) l2 += (int)a.nf[2];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            u1s[t] = rf.y + (float)(NS - t);
+            w1[t] = __builtin_amdgcn_exp2f(-a.coef[1] * u1s[t] * u1s[t]);
+            if (D == 3) {
+                u2s[t] = rf.z + (float)(NS - t);
+                w2[t] = __builtin_amdgcn_exp2f(-a.coef[2] * u2s[t] * u2s[t]);
             }
         }
         for (int64_t coil = 0; coil < a.nc; ++coil) {
@@ -1449,15 +1470,14 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
             // value sums with plain and derivative weights, separably
             float dxr = 0.f, dxi = 0.f, dyr = 0.f, dyi = 0.f, dzr = 0.f, dzi = 0.f;
 #pragma unroll 1
-            for (int t0 = 0; t0 < W; ++t0) {
+            for (int t0 = sub; t0 < W; t0 += tps) {
                 const float u0 = rf.x + (float)(NS - t0);
                 const float w0 = __builtin_amdgcn_exp2f(-a.coef[0] * u0 * u0);
                 int r0 = l0 + t0;
                 if (WRAP && r0 >= R0) r0 -= (int)a.nf[0];
                 float pr = 0.f, pi = 0.f, pyr = 0.f, pyi = 0.f, pzr = 0.f, pzi = 0.f;
+#pragma unroll
                 for (int t1 = 0; t1 < W; ++t1) {
-                    const float u1 = rf.y + (float)(NS - t1);
-                    const float w1 = __builtin_amdgcn_exp2f(-a.coef[1] * u1 * u1);
                     int r1 = l1 + t1;
                     if (WRAP && r1 >= R1) r1 -= (int)a.nf[1];
                     float qr = 0.f, qi = 0.f, qzr = 0.f, qzi = 0.f;
@@ -1465,27 +1485,25 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
                         const int rowb = (r0 * R1 + r1) * R2;
 #pragma unroll
                         for (int t2 = 0; t2 < W; ++t2) {
-                            const float u2 = rf.z + (float)(NS - t2);
-                            const float w2 = __builtin_amdgcn_exp2f(-a.coef[2] * u2 * u2);
                             int r2 = l2 + t2;
                             if (WRAP && r2 >= R2) r2 -= (int)a.nf[2];
                             const float2 gv = lds_gather[rowb + r2];
-                            qr += w2 * gv.x;
-                            qi += w2 * gv.y;
-                            qzr += w2 * u2 * gv.x;
-                            qzi += w2 * u2 * gv.y;
+                            qr += w2[t2] * gv.x;
+                            qi += w2[t2] * gv.y;
+                            qzr += w2[t2] * u2s[t2] * gv.x;
+                            qzi += w2[t2] * u2s[t2] * gv.y;
                         }
                     } else {
                         const float2 gv = lds_gather[r0 * R1 + r1];
                         qr = gv.x;
                         qi = gv.y;
                     }
-                    pr += w1 * qr;
-                    pi += w1 * qi;
-                    pyr += w1 * u1 * qr;
-                    pyi += w1 * u1 * qi;
-                    pzr += w1 * qzr;
-                    pzi += w1 * qzi;
+                    pr += w1[t1] * qr;
+                    pi += w1[t1] * qi;
+                    pyr += w1[t1] * u1s[t1] * qr;
+                    pyi += w1[t1] * u1s[t1] * qi;
+                    pzr += w1[t1] * qzr;
+                    pzi += w1[t1] * qzi;
                 }
                 dxr += w0 * u0 * pr;
                 dxi += w0 * u0 * pi;
@@ -1500,7 +1518,16 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
             g1 += wv.x * dyr + wv.y * dyi;
             g2 += wv.x * dzr + wv.y * dzi;
         }
-        if (valid) {
+        for (int k = 1; k < tps; ++k) {
+            const int src = grp * tps + k;
+            const float o0 = __shfl(g0, src), o1 = __shfl(g1, src), o2 = __shfl(g2, src);
+            if (sub == 0) {
+                g0 += o0;
+                g1 += o1;
+                g2 += o2;
+            }
+        }
+        if (valid && sub == 0) {
             out[rc.w] = g0 * gs0;
             out[a.M + rc.w] = g1 * gs1;
             if (D == 3) out[2 * a.M + rc.w] = g2 * gs2;

@@ -125,6 +125,18 @@ int gn_execute(gn_plan *plan, int kind, const void *in, void *out);
  * they fit the LDS; otherwise the coil images go through a plan buffer). */
 int gn_plan_set_sense(gn_plan *plan, const void *maps);
 
+/* GN_TYPE21 with the intermediate samples in the library's static device scratch
+ * instead of plan memory.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation); the call makes no hipMalloc / hipFree and no allocating library
+ * call.  Limits: M * ncoils <= 4,194,304 (a 32 MiB array per GPU, held until the
+ * process ends; larger problems give GN_EINVAL), the pruned FFT passes (grid lengths
+ * with prime factors 2, 3, 5 and the default engine, else GN_ESTATE) and stage timers
+ * off.  Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (several streams or threads) race; serialize them.  Output equals gn_execute(plan,
+ * GN_TYPE21, in, out) bit for bit. */
+int gn_execute_type21_noalloc(gn_plan *plan, const void *in, void *out);
+
 /* Gradient of the forward map with respect to the sample coordinates
  * (no counterpart in the reference; for autograd): for image f and complex
  * per-sample weights w (device, sample layout of the plan),

@@ -40,6 +40,7 @@ SIGNATURES = {
     "gn_plan_set_stream": ([_vp, _vp], ctypes.c_int),
     "gn_set_points": ([_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
     "gn_execute": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "gn_execute_type21_noalloc": ([_vp, _vp, _vp], ctypes.c_int),
     "gn_plan_set_sense": ([_vp, _vp], ctypes.c_int),
     "gn_fft_c2c": ([ctypes.c_int, _i64p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp, _vp,
                     _vp], ctypes.c_int),

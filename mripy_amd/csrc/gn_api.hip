@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -396,7 +397,13 @@ int gn_set_points(gn_plan *p, int64_t M, const void *x, const void *y, const voi
     return GN_OK;
 }
 
-int gn_execute(gn_plan *p, int kind, const void *in, void *out)
+} // extern "C"
+
+namespace gn {
+
+// gn_execute's body.  ybuf_given: the type-21 intermediate samples (nc * M complex64)
+// provided by the caller instead of the plan's lazily allocated buffer.
+static int execute_impl(gn_plan *p, int kind, const void *in, void *out, float2 *ybuf_given)
 {
     g_err.clear();
     if (!p) return fail(GN_EINVAL, "plan is null");
@@ -500,6 +507,11 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
         break;
     case GN_TYPE21: {
         const int64_t need = p->M * p->nc;
+        if (ybuf_given) {
+            rc = type2(in, ybuf_given);
+            if (!rc) rc = type1(ybuf_given, out, true, 1.0 / (double)p->M);
+            break;
+        }
         if (need > p->ybuf_cap) {
             if (p->ybuf) hipFree(p->ybuf);
             p->ybuf = nullptr;
@@ -513,6 +525,54 @@ int gn_execute(gn_plan *p, int kind, const void *in, void *out)
     }
     }
     return rc;
+}
+
+// Intermediate samples of gn_execute_type21_noalloc: a file-scope device array, one
+// copy per GPU, located with hipGetSymbolAddress once per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20; // 32 MiB = 4,194,304 complex64 samples x coils
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_ptr;
+
+static int scratch_address(void **ptr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void *q = nullptr;
+        e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !q) return hip_fail(e, "hipGetSymbolAddress of the no-allocation scratch");
+        it = g_scratch_ptr.emplace(dev, q).first;
+    }
+    *ptr = it->second;
+    return GN_OK;
+}
+
+} // namespace gn
+
+extern "C" {
+
+int gn_execute(gn_plan *p, int kind, const void *in, void *out) { return execute_impl(p, kind, in, out, nullptr); }
+
+int gn_execute_type21_noalloc(gn_plan *p, const void *in, void *out)
+{
+    g_err.clear();
+    if (!p) return fail(GN_EINVAL, "plan is null");
+    if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called before gn_execute_type21_noalloc");
+    if ((size_t)(p->M * p->nc) * sizeof(float2) > kNoAllocScratchBytes)
+        return fail(GN_EINVAL, "gn_execute_type21_noalloc: M * ncoils exceeds the 4,194,304 complex samples of the "
+                               "static scratch");
+    if (!(p->fft_custom && p->fft_engine == GN_FFT_AUTO))
+        return fail(GN_ESTATE, "gn_execute_type21_noalloc needs the pruned FFT passes (grid lengths with prime "
+                               "factors 2, 3, 5 and the default engine): rocFFT plans allocate");
+    if (p->timing) return fail(GN_ESTATE, "gn_execute_type21_noalloc: stage timers create events; disable them");
+    void *ptr = nullptr;
+    int rc = scratch_address(&ptr);
+    if (rc) return rc;
+    return execute_impl(p, GN_TYPE21, in, out, (float2 *)ptr);
 }
 
 int gn_traj_grad(gn_plan *p, const void *image, const void *weights, float *grad)

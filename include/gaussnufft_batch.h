@@ -61,7 +61,11 @@ int gn_batch_set_graph(gn_batch *batch, int enable);
 int gn_batch_set_tiles(gn_batch *batch, const int *tile);
 
 /* Times `reps` executions of `kind` for each candidate tile configuration and
-   keeps the fastest.  tile (6 ints) and best_ms receive the choice (may be NULL). */
+   keeps the fastest.  Candidates (those that fit the LDS): 3-D spread tiles
+   4x4x8, 4x8x8, 8x8x8, 4x4x16, 8x8x16 (2 to 16 waves of 64-cell blocks per tile)
+   times gather tiles 10x8x12, 10x10x16, 12x12x12, 8x8x16, 8x8x8; 2-D spread tiles
+   8x16, 16x8, 16x16, 8x32 times gather tiles 24x24, 16x32, 32x32, 32x64, 48x48.
+   tile (6 ints) and best_ms receive the choice (may be NULL). */
 int gn_batch_autotune(gn_batch *batch, int kind, const void *const *in, void *const *out, int reps, int *tile,
                       double *best_ms);
 

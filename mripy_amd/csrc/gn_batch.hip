@@ -552,14 +552,16 @@ void destroy(gn_batch_s *b)
 std::vector<std::vector<int>> candidates(int dim)
 {
     std::vector<std::vector<int>> out;
+    // spread tiles set the waves per tile (one wave per 64-cell block, up to 8 per
+    // workgroup); gather tiles set the bin of samples staged together
     if (dim == 2) {
         const int sp[][2] = {{8, 16}, {16, 8}, {16, 16}, {8, 32}};
-        const int ga[][2] = {{24, 24}, {16, 32}, {32, 32}, {32, 64}};
+        const int ga[][2] = {{24, 24}, {16, 32}, {32, 32}, {32, 64}, {48, 48}};
         for (auto &s : sp)
             for (auto &g : ga) out.push_back({s[0], s[1], 1, g[0], g[1], 1});
     } else {
-        const int sp[][3] = {{4, 4, 8}, {4, 8, 8}, {8, 8, 8}, {4, 4, 16}};
-        const int ga[][3] = {{10, 8, 12}, {10, 8, 8}, {8, 8, 16}, {8, 16, 16}};
+        const int sp[][3] = {{4, 4, 8}, {4, 8, 8}, {8, 8, 8}, {4, 4, 16}, {8, 8, 16}};
+        const int ga[][3] = {{10, 8, 12}, {10, 10, 16}, {12, 12, 12}, {8, 8, 16}, {8, 8, 8}};
         for (auto &s : sp)
             for (auto &g : ga) out.push_back({s[0], s[1], s[2], g[0], g[1], g[2]});
     }

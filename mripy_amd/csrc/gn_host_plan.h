@@ -31,8 +31,8 @@ inline void default_tiles(int dim, int ns, const int64_t *nf, int *tile, int *gt
         tile[1] = 4;
         tile[2] = 8;
         gtile[0] = 10;
-        gtile[1] = 8;
-        gtile[2] = 12;
+        gtile[1] = 10;
+        gtile[2] = 16;
     } else {
         tile[0] = 8;
         tile[1] = 16;

@@ -507,7 +507,10 @@ __global__ __launch_bounds__(256) void k_permute_pairs(int64_t n, const int *__r
 
 
 
-constexpr int kSpreadBatch = 64;
+#ifndef GN_SPREAD_BATCH
+#define GN_SPREAD_BATCH 64
+#endif
+constexpr int kSpreadBatch = GN_SPREAD_BATCH; // pairs staged in LDS per step (at most one per lane)
 constexpr int kSpreadWavesMax = 8;
 
 // Spread tile t has no pairs and no later pass reads it: its cells need not be written.

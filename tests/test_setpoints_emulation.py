@@ -23,7 +23,7 @@ KSPREAD_CHUNK = 512
 
 def default_tiles(dim, ns, nf):
     """default_tiles + setup_tiles.fit (gn_kernels.hip, gn_api.hip)."""
-    tile, gtile = ([4, 4, 8], [10, 8, 12]) if dim == 3 else ([8, 16], [24, 24])
+    tile, gtile = ([4, 4, 8], [10, 10, 16]) if dim == 3 else ([8, 16], [24, 24])
 
     def staged():
         r = 16

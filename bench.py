@@ -172,7 +172,13 @@ def leg_3d(cfg, dev, nspread, steps, warmup, tiles=None, keep=False):
     stages = {k: {"ms": v[0] / max(1, v[1]), "launches": v[1]} for k, v in st.items()}
     for k in stages:
         stages[k]["GBps"] = ab[k] / (stages[k]["ms"] * 1e-3) / 1e9 if stages[k]["ms"] > 0 else None
+    # the normal operator (type 21 with density weights 1: type 2, then type 1, one call)
+    from mripy_amd._lib import GN_TYPE21
+    obuf2 = torch.empty_like(obuf)
+    ms_t21 = timed(lambda: P.execute_into(GN_TYPE21, img, obuf2), max(3, steps // 2), 1, stream)
+    del obuf2
     out = {"ms_per_step": ms_step, "value": M * nc / (ms_step * 1e-3), "stages": stages, "ab": ab,
+           "type21_ms": ms_t21,
            "set_points_ms": sorted(sp)[1], "tiles": list(P.tiles), "gather_tiles": list(P.gather_tiles),
            "nf": list(P.nf), "wall_s_timed": wall, "path_bytes": sum(algorithmic_bytes(d, N, G, M, nc).values())}
     if keep:
@@ -242,6 +248,9 @@ def leg_cg_sense(dev):
     b = b / scale
     xi = A.backward(b)
     ms_pair = timed(lambda: A.backward(A.forward(xi)), 10, 3)
+    ms_normal = timed(lambda: A.forward_backward(xi), 10, 3)
+    plain = joint2operators(espirit(torch.as_tensor(cfg["maps"], device=dev)), nuf, fused=False)
+    ms_pair_unfused = timed(lambda: plain.backward(plain.forward(xi)), 10, 3)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     cg = S.prox_l2_Afxnb_CGD2(A.forward, A.backward, b, 30)
@@ -253,7 +262,8 @@ def leg_cg_sense(dev):
     t_admm = time.perf_counter() - t0
     out = {"workload": "256^2 Shepp-Logan x 32 Gaussian-blob coils, 48 x 2048 variable-density spiral, noise 1e-3; "
                        "A = joint2operators(espirit, NUFFT2d), rat 2, nspread 3",
-           "ms_operator_pair": ms_pair, "cg30_s": t_cg, "admm20_s": t_admm,
+           "ms_operator_pair": ms_pair, "ms_operator_pair_unfused_sense": ms_pair_unfused,
+           "ms_forward_backward": ms_normal, "cg30_s": t_cg, "admm20_s": t_admm,
            "coil_samples_per_s_pair": x.shape[0] * 32 / (ms_pair * 1e-3)}
     ref_path = os.path.join(ROOT, "tests", "golden", "solvers_spiral_cs.npz")
     if os.path.exists(ref_path):
@@ -531,6 +541,7 @@ def main():
                      "bytes_per_launch": ab[dom], "avg_launch_ms": stages[dom]["ms"]},
         "stages": stages,
         "set_points_ms": main_leg["set_points_ms"],
+        "type21_ms": main_leg["type21_ms"],
         "wall_s_timed": main_leg["wall_s_timed"],
     }
     keep = main_leg.pop("_keep")

@@ -477,6 +477,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-instance leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the main workload (skip nspread 5, differentiable, CG-SENSE, copy legs)")
+    ap.add_argument("--copy", action="store_true",
+                    help="with --no-extra: still run the 1 GiB copy leg (the counter calibration kernel)")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched uneven-instance leg (prints its JSON object)")
     args = ap.parse_args()
@@ -545,12 +547,13 @@ def main():
         "wall_s_timed": main_leg["wall_s_timed"],
     }
     keep = main_leg.pop("_keep")
-    if not args.no_extra:
+    if not args.no_extra or args.copy:
         log("copy bandwidth")
         cp = copy_bandwidth(dev)
         out["copy"] = cp
         out["roofline"]["frac_vs_copy"] = achieved / (cp["GBps"] * 1e9)
         out["path_roofline"]["frac_vs_copy"] = path_bytes / (ms_step * 1e-3) / (cp["GBps"] * 1e9)
+    if not args.no_extra:
         log("nspread 5")
         l5 = leg_3d(cfg, dev, 5, max(5, args.steps // 2), 2)
         out["nspread5"] = {"workload": "as the main workload at nspread 5 (10 taps/axis; the operator default "

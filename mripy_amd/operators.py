@@ -115,6 +115,25 @@ class _NUFFTnd:
         self.dcf = _flat(dcf)
         self._points_key = None
 
+    def set_params(self, rat=None, nspread=None):
+        """Oversampling ratio and kernel half-width used from the next call on (the
+        reference fixes them through eps=1e-5 in every driver call,
+        pics/operators_cuda_class.py:107-127).  ``None`` keeps the current value.  The
+        plan is rebuilt, and the samples re-binned, at the next operator call; the
+        same limits as the library apply (rat >= 2; nspread 2-16 in 2-D, 2-8 in 3-D)."""
+        rat = self.rat if rat is None else int(rat)
+        nspread = self.nspread if nspread is None else int(nspread)
+        if rat < 2:
+            raise ValueError("rat must be >= 2, got %r" % (rat,))
+        if not 2 <= nspread <= (8 if self.dim == 3 else 16):
+            raise ValueError("nspread %d outside the supported range for %d-D" % (nspread, self.dim))
+        if (rat, nspread) != (self.rat, self.nspread):
+            self.rat, self.nspread = rat, nspread
+            self._plan = None
+            self._plan_key = None
+            self._points_key = None
+        return self
+
     def density_weighting(self, kdata, dcf):
         """kdata * dcf broadcast over trailing dims, squeezed (pics/operators_cuda_class.py:102-105)."""
         ks, ds = dim_match(tuple(kdata.shape), tuple(dcf.shape))

@@ -106,6 +106,13 @@ def test_nufft3d_operator_methods():
     assert rel(op.adjoint(kd), C.type1([x, y, z], kd, p) * M) <= TOL
     # density_weighting squeezes like the reference
     assert op.density_weighting(kd, dcf).shape == (M, nc)
+    # set_params: the next call runs with the new window (and a rebuilt plan)
+    assert (op.rat, op.nspread) == (2, 5)
+    op.set_params(nspread=3)
+    p3 = O.Params((16, 12, 14), rat=2, nspread=3)
+    assert rel(op.forward(im), C.type2([x, y, z], im, p3)) <= TOL
+    with pytest.raises(ValueError):
+        op.set_params(nspread=9)
     kn = op.normalize_set_ktraj(np.stack([x, y, z]))
     assert np.allclose(max(np.max(v) for v in kn), np.pi)
 

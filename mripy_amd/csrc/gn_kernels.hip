@@ -780,8 +780,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
 {
     const int ttp = a.TT <= 16 ? 16 : a.TT <= 32 ? 32 : a.TT <= 64 ? 64 : 128; // table row stride
-    const size_t batch = (size_t)(kSpreadBatch + 1) * ttp * 4 + (size_t)(kSpreadBatch + 1) * 16 * 4 +
-                         (size_t)kSpreadBatch * 16 + (size_t)kSpreadBatch * 64 + (size_t)waves * 4 * kSpreadBatch * 4;
+    const size_t batch = (size_t)(kSpreadBatch + 1) * ttp * 4 * 2 + (size_t)kSpreadBatch * 16 +
+                         (size_t)kSpreadBatch * 64 + (size_t)waves * 4 * kSpreadBatch * 4;
     const size_t out = (size_t)8 * (a.TC + 1) * 8; // output staging: up to 8 coils of the tile
     return std::max(batch, out);
 }
@@ -792,16 +792,18 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     constexpr int B = kSpreadBatch;
     extern __shared__ float lds_sp[];
     if (tile_skippable(a, item)) return; // uniform over the workgroup
-    // LDS: weight tables [B + 1][ttp] | reals [B + 1][16] | masks [B][4] | per-axis window
-    //      parameters [B][4] {l, c, ns - j, t} | per-wave pair lists [waves][4][B];
-    //      row B of the tables and reals is the all-zero pair
+    // LDS: weight tables [B + 1][ttp] | reals [B + 1][ttp] (16 used) | masks [B][4] |
+    //      per-axis window parameters [B][4] {l, c, ns - j, t} | per-wave pair lists
+    //      [waves][4][B]; row B of the tables and reals is the all-zero pair.  Table and
+    //      value rows have the same stride, so a list entry is the byte offset of a
+    //      pair's rows in both.
     const int TT = a.TT;
     // table rows padded to a power of two: a pair's row starts at kk << lgt
     const int ttp = TT <= 16 ? 16 : TT <= 32 ? 32 : TT <= 64 ? 64 : 128;
     const int lgt = __builtin_ctz(ttp);
     float *s_tab = lds_sp;
     float *s_val = s_tab + (B + 1) * ttp;
-    int *s_msk = (int *)(s_val + (B + 1) * 16);
+    int *s_msk = (int *)(s_val + (B + 1) * ttp);
     int4 *s_par = (int4 *)(s_msk + B * 4);
     int *s_list = (int *)(s_par + B * 4);
 
@@ -845,9 +847,19 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     }
     const int cz = to2 + Zc;
     const uint32_t sbm2 = D == 3 ? (0xfu << (bq2 * 4)) : 1u;
+    // this lane's table columns and value in the rows of pair 0; a list entry (byte
+    // offset of a pair's rows) added to them addresses the pair's entries
+    const char *px[4], *py[4];
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+        px[sb] = (const char *)(s_tab + cx[sb]);
+        py[sb] = (const char *)(s_tab + cy[sb]);
+    }
+    const char *pz = (const char *)(s_tab + cz);
+    const char *pv = (const char *)(s_val + ri);
 
     if (tid < ttp) s_tab[B * ttp + tid] = 0.f;
-    if (tid < 16) s_val[B * 16 + tid] = 0.f;
+    if (tid < 16) s_val[B * ttp + tid] = 0.f;
 
     float S[4][4], K[4][4];
 #pragma unroll
@@ -855,12 +867,14 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
 #pragma unroll
         for (int v = 0; v < 4; ++v) S[sb][v] = K[sb][v] = 0.f;
 
-    // phase-2 entry of this thread: table column e_i (axis e_ax, cell e_L) of pairs
-    // e_k0, e_k0 + e_kstep, ...; columns are padded to a power of two (the workgroup
-    // has at least 64 threads and TT <= 96, so at least one pair per step)
-    const int e_i = tid & (ttp - 1);
-    const int e_k0 = tid / ttp;
-    const int e_kstep = max(1, (int)blockDim.x / ttp);
+    // phase-2 entry of this thread: the 4 consecutive table columns e_i .. e_i + 3 (one
+    // axis e_ax, cells e_L .. e_L + 3: the tile extents are multiples of 4) of pairs
+    // e_k0, e_k0 + e_kstep, ...; a row holds ttp / 4 column quads (ttp <= 128, so a
+    // workgroup of at least 64 threads covers at least two pairs per step)
+    const int lgq = lgt - 2;
+    const int e_i = (tid & ((ttp >> 2) - 1)) << 2;
+    const int e_k0 = tid >> lgq;
+    const int e_kstep = (int)blockDim.x >> lgq;
     const int e_ax = e_i < to1 ? 0 : (e_i < to2 ? 1 : 2);
     const int e_L = e_i - (e_ax == 0 ? 0 : (e_ax == 1 ? to1 : to2));
     const int e_T = e_ax == 0 ? T0 : (e_ax == 1 ? T1 : T2);
@@ -899,7 +913,7 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
                     }
                 }
             }
-            float4 *dv = (float4 *)(s_val + tid * 16);
+            float4 *dv = (float4 *)(s_val + tid * ttp);
 #pragma unroll
             for (int c2 = 0; c2 < 4; ++c2) {
                 float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -927,18 +941,22 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         }
         __syncthreads();
         // phase 2: weight tables, one entry per (pair, axis, tile-local cell); a thread
-        // keeps one entry index i (its axis and cell) and steps over the pairs
+        // keeps one column quad (its axis and 4 cells) and steps over the pairs
         if (e_i < TT) {
             for (int k = e_k0; k < n; k += e_kstep) {
                 const int4 par = s_par[k * 4 + e_ax];
-                int rel = e_L - par.x;
-                if (rel < 0) rel += e_T;
-                float w = 0.f;
-                if (rel < par.y) {
+                int rel0 = e_L - par.x; // (L - l) mod T of the quad's first column
+                if (rel0 < 0) rel0 += e_T;
+                float w[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    int rel = rel0 + i;
+                    if (i > 0 && rel >= e_T) rel -= e_T;
                     const float u = __int_as_float(par.w) + (float)(par.z - rel);
-                    w = __builtin_amdgcn_exp2f(-e_coef * u * u);
+                    const float g = __builtin_amdgcn_exp2f(-e_coef * u * u);
+                    w[i] = rel < par.y ? g : 0.f;
                 }
-                s_tab[(k << lgt) + e_i] = w;
+                *(float4 *)(s_tab + (k << lgt) + e_i) = make_float4(w[0], w[1], w[2], w[3]);
             }
         }
         __syncthreads();
@@ -961,13 +979,13 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
             cmax = max(cmax, (cnt[sb] + 3) & ~3);
             int *lst = s_list + (wave * 4 + sb) * B;
             const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0));
-            if (hit) lst[pos] = lane;
+            if (hit) lst[pos] = lane << (lgt + 2);
         }
         // every list padded with the zero pair to the longest: the four sub-blocks'
         // products then run interleaved, their LDS reads in flight together
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb)
-            if (lane >= cnt[sb] && lane < cmax) s_list[(wave * 4 + sb) * B + lane] = B;
+            if (lane >= cnt[sb] && lane < cmax) s_list[(wave * 4 + sb) * B + lane] = B << (lgt + 2);
         // the lists were written by this wave's own lanes: order the LDS accesses
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         f32x4 acc[4];
@@ -977,23 +995,30 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         for (int t = 0; t < cmax; t += 4) {
 #pragma unroll
             for (int sb = 0; sb < 4; ++sb) {
-                const int kk = lst0[sb * B + t];
-                const float *tab = s_tab + (kk << lgt);
-                float w = tab[cx[sb]] * tab[cy[sb]];
-                if (D == 3) w *= tab[cz];
-                const float v = s_val[(kk << 4) + ri];
+                const int off = lst0[sb * B + t]; // byte offset of the pair's rows
+                float w = *(const float *)(px[sb] + off) * *(const float *)(py[sb] + off);
+                if (D == 3) w *= *(const float *)(pz + off);
+                const float v = *(const float *)(pv + off);
                 acc[sb] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, v, acc[sb], 0, 0, 0);
             }
         }
+        if (base == item.y) {
+            // first batch: the Kahan step from zero sums is exact (S = acc, K = 0)
 #pragma unroll
-        for (int sb = 0; sb < 4; ++sb)
+            for (int sb = 0; sb < 4; ++sb)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const float y = acc[sb][v] - K[sb][v];
-                const float tsum = S[sb][v] + y;
-                K[sb][v] = (tsum - S[sb][v]) - y;
-                S[sb][v] = tsum;
-            }
+                for (int v = 0; v < 4; ++v) S[sb][v] = acc[sb][v];
+        } else {
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float y = acc[sb][v] - K[sb][v];
+                    const float tsum = S[sb][v] + y;
+                    K[sb][v] = (tsum - S[sb][v]) - y;
+                    S[sb][v] = tsum;
+                }
+        }
     }
     // write through LDS: every wave stores its sub-blocks' sums as [coil][cell]
     // (a cell of padding per coil), then the workgroup writes the tile row by row
@@ -1241,6 +1266,84 @@ __device__ inline void stage_region_pair(float4 *lds, const float2 *__restrict__
     }
 }
 
+// Row-wise staging pays when whole rows fill at least 7/8 of a wave's lanes (21 or
+// 29 cells: 63 or 58 lanes; 25 cells would leave 14 idle) and grid indices fit 31 bits.
+__device__ inline bool rows_stage_ok(int RL, int64_t G)
+{
+    return RL <= 64 && (64 / RL) * RL >= 56 && G < 0x7fffffffll;
+}
+
+// Row-wise staging of the same region for rows of at most 64 cells along the last
+// (contiguous) axis and grids of fewer than 2^31 cells per coil.  Each wave takes
+// 64 / RL whole rows per step (lane = row slot * RL + cell), so a lane keeps one
+// last-axis cell for the whole loop (its wrapped grid index is computed once) and
+// only the leading-axis coordinates of its row advance, with one carry; grid indices
+// are 32-bit.  A wave's lanes write consecutive LDS cells.  PAIR: 16-byte cells
+// holding coils g0 and g1 (zeros for a missing second coil); else 8-byte cells of g0.
+template <int D, bool PAIR, typename Cell>
+__device__ inline void stage_region_rows(Cell *lds, const float2 *__restrict__ g0, const float2 *__restrict__ g1,
+                                         bool two, const int64_t *nf, const int *org, int R0, int R1, int R2)
+{
+    constexpr int U = 4; // rows per lane whose loads are in flight together
+    const int RL = D == 3 ? R2 : R1;
+    const int nrow = D == 3 ? R0 * R1 : R0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rpw = 64 / RL;
+    const int k = lane / RL, c = lane - k * RL;
+    if (k >= rpw) return;
+    const int step = (int)(blockDim.x >> 6) * rpw;
+    const int nf0 = (int)nf[0], nf1 = (int)nf[1], nfl = (int)nf[D - 1];
+    int ql = org[D - 1] + c;
+    if (ql >= nfl) ql -= nfl;
+    int rho = wave * rpw + k;
+    int r0, r1 = 0, s0 = step, s1 = 0;
+    if (D == 3) {
+        r0 = rho / R1;
+        r1 = rho - r0 * R1;
+        s0 = step / R1;
+        s1 = step - s0 * R1;
+    } else {
+        r0 = rho;
+    }
+    const uint32_t nf2 = D == 3 ? (uint32_t)nf[2] : 1u;
+    for (; rho < nrow; rho += U * step) {
+        Cell v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (rho + u * step < nrow) {
+                int q0 = org[0] + r0;
+                if (q0 >= nf0) q0 -= nf0;
+                uint32_t off;
+                if (D == 3) {
+                    int q1 = org[1] + r1;
+                    if (q1 >= nf1) q1 -= nf1;
+                    off = ((uint32_t)q0 * (uint32_t)nf1 + (uint32_t)q1) * nf2 + (uint32_t)ql;
+                } else {
+                    off = (uint32_t)q0 * (uint32_t)nf1 + (uint32_t)ql;
+                }
+                const float2 x0 = g0[off];
+                if constexpr (PAIR) {
+                    const float2 x1 = two ? g1[off] : make_float2(0.f, 0.f);
+                    v[u] = make_float4(x0.x, x0.y, x1.x, x1.y);
+                } else {
+                    v[u] = x0;
+                }
+            }
+            if (D == 3) {
+                r1 += s1;
+                const int cc = r1 >= R1 ? 1 : 0;
+                r1 -= cc * R1;
+                r0 += s0 + cc;
+            } else {
+                r0 += step;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (rho + u * step < nrow) lds[(rho + u * step) * RL + c] = v[u];
+    }
+}
+
 // acc += w * v on both coils of a staged cell
 __device__ inline void fma4(float4 &acc, float w, const float4 v)
 {
@@ -1274,7 +1377,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int4 item
     const int RC = R0 * R1 * R2;
 
     const float2 *g0 = a.grid + (int64_t)cg0 * a.G;
-    stage_region_pair<D>(lds_pair, g0, g0 + a.G, two, a.nf, org, R1, R2, RC);
+    if (rows_stage_ok(D == 3 ? R2 : R1, a.G))
+        stage_region_rows<D, true>(lds_pair, g0, g0 + a.G, two, a.nf, org, R0, R1, R2);
+    else
+        stage_region_pair<D>(lds_pair, g0, g0 + a.G, two, a.nf, org, R1, R2, RC);
     __syncthreads();
     // Threads per sample: a work item's samples are spread over all waves of the
     // workgroup.  The tps lanes of a group share one sample and take the leading-axis
@@ -1467,7 +1573,10 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
         for (int64_t coil = 0; coil < a.nc; ++coil) {
             const float2 *g = a.grid + coil * a.G;
             __syncthreads();
-            stage_region<D>(lds_gather, g, a.nf, org, R1, R2, RC);
+            if (rows_stage_ok(D == 3 ? R2 : R1, a.G))
+                stage_region_rows<D, false>(lds_gather, g, g, false, a.nf, org, R0, R1, R2);
+            else
+                stage_region<D>(lds_gather, g, a.nf, org, R1, R2, RC);
             __syncthreads();
             if (!valid) continue;
             // value sums with plain and derivative weights, separably

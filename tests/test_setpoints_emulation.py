@@ -218,3 +218,86 @@ def test_wide_window_default_gather_tile_fits_lds():
         tile, gtile = default_tiles(3, ns, nf)
         staged = 16 * np.prod([min(g + 2 * ns - 1, n) for g, n in zip(gtile, nf)])
         assert staged <= 160 * 1024, (ns, gtile)
+
+
+def stage_rows_emulated(region, org, nf, nthreads):
+    """stage_region_rows (gn_kernels.hip): the LDS cell and grid index every lane writes,
+    stepping rows exactly as the kernel does (carry on the second axis, U rows per pass)."""
+    dim = len(nf)
+    R0, R1 = region[0], region[1]
+    R2 = region[2] if dim == 3 else 1
+    RL = R2 if dim == 3 else R1
+    nrow = R0 * R1 if dim == 3 else R0
+    U = 4
+    writes = []
+    for tid in range(nthreads):
+        lane, wave = tid & 63, tid >> 6
+        rpw = 64 // RL
+        k, c = lane // RL, lane % RL
+        if k >= rpw:
+            continue
+        step = (nthreads >> 6) * rpw
+        ql = org[dim - 1] + c
+        if ql >= nf[dim - 1]:
+            ql -= nf[dim - 1]
+        rho = wave * rpw + k
+        if dim == 3:
+            r0, r1 = rho // R1, rho % R1
+            s0, s1 = step // R1, step % R1
+        else:
+            r0, r1, s0, s1 = rho, 0, step, 0
+        while rho < nrow:
+            for u in range(U):
+                if rho + u * step < nrow:
+                    q0 = org[0] + r0
+                    if q0 >= nf[0]:
+                        q0 -= nf[0]
+                    assert 0 <= q0 < nf[0]
+                    if dim == 3:
+                        q1 = org[1] + r1
+                        if q1 >= nf[1]:
+                            q1 -= nf[1]
+                        assert 0 <= q1 < nf[1]
+                        off = (q0 * nf[1] + q1) * nf[2] + ql
+                    else:
+                        off = q0 * nf[1] + ql
+                    assert 0 <= off < min(int(np.prod(nf)), 2 ** 31 - 1)
+                    writes.append(((rho + u * step) * RL + c, off))
+                if dim == 3:
+                    r1 += s1
+                    cc = 1 if r1 >= R1 else 0
+                    r1 -= cc * R1
+                    r0 += s0 + cc
+                else:
+                    r0 += step
+            rho += U * step
+    return writes
+
+
+@pytest.mark.parametrize("modes,ns,rat", [((128, 128, 128), 3, 2), ((128, 128, 128), 5, 2), ((16, 20, 18), 3, 2),
+                                          ((24, 20, 18), 5, 2), ((8, 8, 8), 6, 2), ((128, 128), 3, 2),
+                                          ((40, 36), 6, 3), ((7, 9), 3, 2)])
+def test_row_staging_covers_region_once(modes, ns, rat):
+    """Every cell of a gather region is staged exactly once, at the wrapped grid index
+    of its region coordinates, for tiles at the origin, in the interior and at the
+    upper edge of every axis (wrap-around), with 512-thread workgroups."""
+    dim = len(modes)
+    nf = [max(rat * m, 2 * ns) for m in modes]
+    _, gtile = default_tiles(dim, ns, nf)
+    region = [min(gtile[a] + 2 * ns - 1, nf[a]) for a in range(dim)]
+    RL = region[-1]
+    if RL > 64 or (64 // RL) * RL < 56:
+        pytest.skip("rows that fill less than 7/8 of a wave use the flat staging loop")
+    ng = [-(-nf[a] // gtile[a]) for a in range(dim)]
+    for pick in ("first", "middle", "last"):
+        org = [{"first": 0, "middle": ng[a] // 2, "last": ng[a] - 1}[pick] * gtile[a] for a in range(dim)]
+        w = stage_rows_emulated(region, org, nf, 512)
+        RC = int(np.prod(region))
+        cells = np.array([x[0] for x in w])
+        assert len(cells) == RC and np.array_equal(np.sort(cells), np.arange(RC))
+        want = {}
+        for r in np.ndindex(*region):
+            q = [(org[a] + r[a]) % nf[a] for a in range(dim)]
+            lin = int(np.ravel_multi_index(r, region))
+            want[lin] = int(np.ravel_multi_index(q, nf))
+        assert all(want[cidx] == off for cidx, off in w)

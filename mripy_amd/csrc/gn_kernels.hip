@@ -757,15 +757,16 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const Spr
 //
 // Same work items, pair staging and weight tables as spread_body; the per-wave
 // visit loop becomes small dense products.  A wave's 64-cell block is split into
-// four sub-blocks of 16 cells (2x2x4 in 3-D, 4x4 in 2-D).  Per batch, the wave
-// lists, for each sub-block, the staged pairs whose windows reach it (padded to
-// a multiple of 4 with an all-zero pair) and accumulates
+// four slices of 16 cells (in 3-D the 4x4 x-y cells of each of the block's 4 z
+// planes; in 2-D four 4x4 sub-blocks) and accumulates
 //     D[cell i][real j] += sum over 4 listed pairs k of  w_k(cell i) * v_k[j]
-// with one MFMA per 4 pairs: rows are the sub-block's cells, columns the 2*NCG
-// reals (re, im per coil; zero columns beyond), K the pairs.  The four lists are
-// padded to the longest so the four sub-blocks' products run interleaved.  A
-// sub-block covers on average ~half its cells per visit (a 64-cell block ~30%),
-// and the FMAs run on the matrix pipe.  The products are summed in a fixed order,
+// with one MFMA per slice and 4 pairs: rows are the slice's cells, columns the
+// 2*NCG reals (re, im per coil; zero columns beyond), K the pairs.  Per batch the
+// wave lists the staged pairs that reach it (padded to a multiple of 4 with an
+// all-zero pair): in 3-D one list for the block, each step forming a lane's x-y
+// weight once for the four z-slice MFMAs; in 2-D one list per sub-block, padded to
+// the longest so the four products run interleaved.  The FMAs run on the matrix
+// pipe.  The products are summed in a fixed order,
 // so the result is deterministic; batch sums are folded into Kahan-compensated
 // registers as in spread_body.  Lane layout of the 16x16x4 f32 operation:
 // A[i][k] and B[k][j] in lane i + 16k and j + 16k, D[4g + v][j] in register v of
@@ -855,8 +856,12 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         px[sb] = (const char *)(s_tab + cx[sb]);
         py[sb] = (const char *)(s_tab + cy[sb]);
     }
-    const char *pz = (const char *)(s_tab + cz);
     const char *pv = (const char *)(s_val + ri);
+    // 3-D: the lane's x and y table columns (row cell ri of the 4x4x4 block) and the
+    // block's 4 z columns (16-byte aligned: the z table starts at T0 + T1, a multiple of 4)
+    const char *qx = (const char *)(s_tab + bq0 * 4 + (ri >> 2));
+    const char *qy = (const char *)(s_tab + to1 + bq1 * 4 + (ri & 3));
+    const char *qz = (const char *)(s_tab + to2 + bq2 * 4);
 
     if (tid < ttp) s_tab[B * ttp + tid] = 0.f;
     if (tid < 16) s_val[B * ttp + tid] = 0.f;
@@ -961,13 +966,46 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         }
         __syncthreads();
         if (!has_block) continue;
-        // phase 3: per sub-block, list the pairs that reach it, then 4 pairs per MFMA
         uint32_t m0 = 0, m1 = 0, m2 = 0;
         if (lane < n) {
             m0 = (uint32_t)s_msk[lane * 4];
             m1 = (uint32_t)s_msk[lane * 4 + 1];
             m2 = D == 3 ? (uint32_t)s_msk[lane * 4 + 2] : 1u;
         }
+        f32x4 acc[4];
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) acc[sb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (D == 3) {
+            // phase 3 (3-D): one list of the pairs whose windows reach the wave's 4x4x4
+            // block.  The 16 rows of an MFMA are the block's x-y cells (lane row ri:
+            // x = ri >> 2, y = ri & 3) and the four z slices are four MFMAs: per 4
+            // listed pairs the lane forms its x-y weight once and multiplies it by the
+            // pair's 4 z weights of the block (one 16-byte read).  A pair that misses
+            // a slice adds exact zeros there.
+            const bool hit = (m0 & (0xfu << (bq0 * 4))) && (m1 & (0xfu << (bq1 * 4))) && (m2 & sbm2);
+            const unsigned long long hm = __ballot(hit);
+            const int cnt = __popcll(hm);
+            const int cpad = (cnt + 3) & ~3;
+            int *lst = s_list + wave * 4 * B;
+            const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0));
+            if (hit) lst[pos] = lane << (lgt + 2);
+            if (lane >= cnt && lane < cpad) lst[lane] = B << (lgt + 2);
+            // the list was written by this wave's own lanes: order the LDS accesses
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const int *lst0 = lst + kg;
+            for (int t = 0; t < cpad; t += 4) {
+                const int off = lst0[t]; // byte offset of the pair's rows
+                const float wxy = *(const float *)(qx + off) * *(const float *)(qy + off);
+                const float4 wz = *(const float4 *)(qz + off);
+                const float v = *(const float *)(pv + off);
+                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.x, v, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.y, v, acc[1], 0, 0, 0);
+                acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.z, v, acc[2], 0, 0, 0);
+                acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.w, v, acc[3], 0, 0, 0);
+            }
+        } else {
+        // phase 3 (2-D): per 4x4 sub-block, list the pairs that reach it, then 4 pairs
+        // per MFMA
         int cnt[4], cmax = 0;
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
@@ -988,19 +1026,16 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
             if (lane >= cnt[sb] && lane < cmax) s_list[(wave * 4 + sb) * B + lane] = B << (lgt + 2);
         // the lists were written by this wave's own lanes: order the LDS accesses
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        f32x4 acc[4];
-#pragma unroll
-        for (int sb = 0; sb < 4; ++sb) acc[sb] = f32x4{0.f, 0.f, 0.f, 0.f};
         const int *lst0 = s_list + wave * 4 * B + kg;
         for (int t = 0; t < cmax; t += 4) {
 #pragma unroll
             for (int sb = 0; sb < 4; ++sb) {
                 const int off = lst0[sb * B + t]; // byte offset of the pair's rows
-                float w = *(const float *)(px[sb] + off) * *(const float *)(py[sb] + off);
-                if (D == 3) w *= *(const float *)(pz + off);
+                const float w = *(const float *)(px[sb] + off) * *(const float *)(py[sb] + off);
                 const float v = *(const float *)(pv + off);
                 acc[sb] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, v, acc[sb], 0, 0, 0);
             }
+        }
         }
         if (base == item.y) {
             // first batch: the Kahan step from zero sums is exact (S = acc, K = 0)
@@ -1033,14 +1068,16 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     if (!empty && has_block && (ri >> 1) < ncg) {
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
-            const int X = (D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1);
-            const int Y = (D == 3 ? bq1 * 4 : bq1 * 8) + SB1 * (sb & 1);
+            // 3-D: sb is the z slice and register v of lane j + 16 kg holds row 4 kg + v,
+            // i.e. x = kg, y = v; 2-D: sb is the 4x4 sub-block, row 4 kg + v its cell
+            const int X = D == 3 ? bq0 * 4 : bq0 * 8 + SB0 * (sb >> 1);
+            const int Y = D == 3 ? bq1 * 4 : bq1 * 8 + SB1 * (sb & 1);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int cell = 4 * kg + v;
-                const int L0 = X + (D == 3 ? (cell >> 3) : (cell >> 2));
-                const int L1 = Y + (D == 3 ? ((cell >> 2) & 1) : (cell & 3));
-                const int L2 = D == 3 ? bq2 * 4 + (cell & 3) : 0;
+                const int L0 = X + (D == 3 ? kg : (cell >> 2));
+                const int L1 = Y + (D == 3 ? v : (cell & 3));
+                const int L2 = D == 3 ? bq2 * 4 + sb : 0;
                 s_out[(ri >> 1) * CS + ((L0 * T1 + L1) * T2 + L2) * 2 + (ri & 1)] = S[sb][v] - K[sb][v];
             }
         }

@@ -1390,6 +1390,26 @@ __device__ inline void fma4(float4 &acc, float w, const float4 v)
     acc.w += w * v.w;
 }
 
+// Lanes of the gather-type kernels when tps lanes share a sample: sample grp of the
+// wave, tap share sub; a sample's other lanes are lane + k * stride (k < tps - sub).
+// Wide windows (nspread >= 4) space a sample's lanes spw apart, so lanes 0 .. spw-1
+// hold consecutive samples and the LDS-bank order of the records (k_bank_order)
+// holds for every tps (nspread 5: gather 1.86 -> 1.79 ms); at nspread 3 adjacent
+// lanes per sample measured 1% faster.
+template <int NS>
+__device__ inline void lane_layout(int lane, int tps, int spw, int *grp, int *sub, int *stride)
+{
+    if (NS >= 4) {
+        *sub = lane / spw;
+        *grp = lane - *sub * spw;
+        *stride = spw;
+    } else {
+        *grp = lane / tps;
+        *sub = lane - *grp * tps;
+        *stride = 1;
+    }
+}
+
 // Tap weights use the hardware exp2 (v_exp_f32) directly.  The exponent is
 // coef * u^2 with coef = pi (rat - 1/2) / (rat ns ln 2) and |u| <= ns + 1, so every
 // weight is >= 2^-82 for nspread <= 16: far from the denormal range, where the
@@ -1420,9 +1440,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int4 item
         stage_region_pair<D>(lds_pair, g0, g0 + a.G, two, a.nf, org, R1, R2, RC);
     __syncthreads();
     // Threads per sample: a work item's samples are spread over all waves of the
-    // workgroup.  The tps lanes of a group share one sample and take the leading-axis
-    // taps t0 = sub, sub + tps, ... (tps divides W); lane sub 0 then adds the partial
-    // sums of lanes 1 .. tps-1 in that order, so the result does not depend on timing.
+    // workgroup.  The tps lanes of a sample take the leading-axis taps t0 = sub,
+    // sub + tps, ... (tps divides W); lane sub 0 then adds the partial sums of lanes
+    // sub 1 .. tps-1 in that order, so the result does not depend on timing or on the
+    // lane layout (lane_layout).
     const int n = item.z - item.y;
     const int nwaves = kGatherThreads / 64;
     int tps = 1;
@@ -1431,10 +1452,11 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int4 item
         if (W % d == 0 && n <= nwaves * (64 / d)) tps = d;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int spw = 64 / tps;               // samples per wave
-    const int grp = lane / tps, sub = lane - grp * tps;
+    int grp, sub, stride;
+    lane_layout<NS>(lane, tps, spw, &grp, &sub, &stride);
     for (int s0 = 0; s0 < n; s0 += nwaves * spw) {
         const int slot = s0 + wave * spw + grp;
-        const bool valid = grp < spw && slot < n;
+        const bool valid = grp < spw && sub < tps && slot < n;
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
         int orig = 0;
         if (valid) {
@@ -1489,9 +1511,9 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int4 item
                 fma4(acc, w0, p);
             }
         }
-        // every lane takes part in the exchange (groups never straddle a wave)
+        // every lane takes part in the exchange (a sample's lanes are in one wave)
         for (int k = 1; k < tps; ++k) {
-            const int src = grp * tps + k;
+            const int src = lane + k * stride;
             const float4 o = make_float4(__shfl(acc.x, src), __shfl(acc.y, src), __shfl(acc.z, src),
                                          __shfl(acc.w, src));
             if (sub == 0) {
@@ -1577,10 +1599,11 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
         if (W % d == 0 && n <= nwaves * (64 / d)) tps = d;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int spw = 64 / tps;
-    const int grp = lane / tps, sub = lane - grp * tps;
+    int grp, sub, stride;
+    lane_layout<NS>(lane, tps, spw, &grp, &sub, &stride);
     for (int s0 = 0; s0 < n; s0 += nwaves * spw) {
         const int slot = s0 + wave * spw + grp;
-        const bool valid = grp < spw && slot < n;
+        const bool valid = grp < spw && sub < tps && slot < n;
         float g0 = 0.f, g1 = 0.f, g2 = 0.f;
         int4 rc = make_int4(0, 0, 0, 0);
         float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1668,7 +1691,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_traj_grad(GatherArgs a, cons
             g2 += wv.x * dzr + wv.y * dzi;
         }
         for (int k = 1; k < tps; ++k) {
-            const int src = grp * tps + k;
+            const int src = lane + k * stride;
             const float o0 = __shfl(g0, src), o1 = __shfl(g1, src), o2 = __shfl(g2, src);
             if (sub == 0) {
                 g0 += o0;

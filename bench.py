@@ -264,6 +264,8 @@ def leg_cg_sense(dev):
                        "A = joint2operators(espirit, NUFFT2d), rat 2, nspread 3",
            "ms_operator_pair": ms_pair, "ms_operator_pair_unfused_sense": ms_pair_unfused,
            "ms_forward_backward": ms_normal, "cg30_s": t_cg, "admm20_s": t_admm,
+           "note": "cg30_s: prox_l2_Afxnb_CGD2 with Nite=30, which ends early at the reference's stop test "
+                   "(delta <= 1e-3 delta0, pics/proximal_func.py:314-343); admm20_s: 20 outer ADMM-TV iterations",
            "coil_samples_per_s_pair": x.shape[0] * 32 / (ms_pair * 1e-3)}
     ref_path = os.path.join(ROOT, "tests", "golden", "solvers_spiral_cs.npz")
     if os.path.exists(ref_path):

@@ -235,9 +235,18 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
     int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
     load_table(tw, a.tw, a.L, a.sign);
     for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
-    if (a.rows_in < a.L) // pruned input: zero every slot first
-        for (int e = threadIdx.x; e < a.L * estride; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
+    // pruned input: the FFT positions no input row maps to, [rows_in - half_in,
+    // L - half_in) with in_map and [rows_in, L) without, start as zeros; the loads
+    // below write every other slot, so no barrier separates the two
+    if (a.rows_in < a.L) {
+        const int pz = a.in_map ? a.rows_in - half_in : a.rows_in;
+        const int nz = (a.L - a.rows_in) << cbs;
+        for (int e = threadIdx.x; e < nz; e += blockDim.x) {
+            const int r = e >> cbs, j = e & (CB - 1);
+            buf[spinv[pz + r] * estride + j] = make_float2(0.f, 0.f);
+        }
+    }
     // kLoadUnroll independent global loads in flight per thread before the LDS stores
     for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
         float4 v[kLoadUnroll];

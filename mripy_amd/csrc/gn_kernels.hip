@@ -756,17 +756,17 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const Spr
 // Adjoint spread on the matrix cores (v_mfma_f32_16x16x4_f32).
 //
 // Same work items, pair staging and weight tables as spread_body; the per-wave
-// visit loop becomes small dense products.  A wave's 64-cell block is split into
-// four slices of 16 cells (in 3-D the 4x4 x-y cells of each of the block's 4 z
-// planes; in 2-D four 4x4 sub-blocks) and accumulates
+// visit loop becomes small dense products.  A wave's 64-cell block (4x4x4 in 3-D,
+// 8x8 in 2-D) is split into four slices of 16 cells (in 3-D the 4x4 x-y cells of
+// each of the block's 4 z planes; in 2-D pairs of x rows of 8 cells) and
+// accumulates
 //     D[cell i][real j] += sum over 4 listed pairs k of  w_k(cell i) * v_k[j]
 // with one MFMA per slice and 4 pairs: rows are the slice's cells, columns the
 // 2*NCG reals (re, im per coil; zero columns beyond), K the pairs.  Per batch the
-// wave lists the staged pairs that reach it (padded to a multiple of 4 with an
-// all-zero pair): in 3-D one list for the block, each step forming a lane's x-y
-// weight once for the four z-slice MFMAs; in 2-D one list per sub-block, padded to
-// the longest so the four products run interleaved.  The FMAs run on the matrix
-// pipe.  The products are summed in a fixed order,
+// wave lists the staged pairs that reach its block (padded to a multiple of 4 with
+// an all-zero pair); each step a lane forms the weight factor its row shares across
+// the slices once (3-D: the x-y weight; 2-D: the y weight) and multiplies it by the
+// per-slice factor.  The FMAs run on the matrix pipe.  The products are summed in a fixed order,
 // so the result is deterministic; batch sums are folded into Kahan-compensated
 // registers as in spread_body.  Lane layout of the 16x16x4 f32 operation:
 // A[i][k] and B[k][j] in lane i + 16k and j + 16k, D[4g + v][j] in register v of
@@ -782,7 +782,7 @@ inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
 {
     const int ttp = a.TT <= 16 ? 16 : a.TT <= 32 ? 32 : a.TT <= 64 ? 64 : 128; // table row stride
     const size_t batch = (size_t)(kSpreadBatch + 1) * ttp * 4 * 2 + (size_t)kSpreadBatch * 16 +
-                         (size_t)kSpreadBatch * 64 + (size_t)waves * 4 * kSpreadBatch * 4;
+                         (size_t)kSpreadBatch * 64 + (size_t)waves * kSpreadBatch * 4;
     const size_t out = (size_t)8 * (a.TC + 1) * 8; // output staging: up to 8 coils of the tile
     return std::max(batch, out);
 }
@@ -795,7 +795,7 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     if (tile_skippable(a, item)) return; // uniform over the workgroup
     // LDS: weight tables [B + 1][ttp] | reals [B + 1][ttp] (16 used) | masks [B][4] |
     //      per-axis window parameters [B][4] {l, c, ns - j, t} | per-wave pair lists
-    //      [waves][4][B]; row B of the tables and reals is the all-zero pair.  Table and
+    //      [waves][B]; row B of the tables and reals is the all-zero pair.  Table and
     //      value rows have the same stride, so a list entry is the byte offset of a
     //      pair's rows in both.
     const int TT = a.TT;
@@ -831,36 +831,22 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         bq0 = fdiv_small(bidx, a.nblk[1], 1.0f / (float)a.nblk[1]);
         bq1 = bidx - bq0 * a.nblk[1];
     }
-    // this lane's row (cell) within a sub-block and its pair slot k = lane / 16
+    // this lane's MFMA row ri (a cell of a slice) and its pair slot k = lane / 16
     const int ri = lane & 15, kg = lane >> 4;
-    // tile-local origin of sub-block 0 and the row's offset inside a sub-block;
-    // sub-block s adds (SB0 * (s >> 1), SB1 * (s & 1)) on axes 0 and 1
-    constexpr int SB0 = D == 3 ? 2 : 4, SB1 = D == 3 ? 2 : 4;
-    const int X0 = (D == 3 ? bq0 * 4 : bq0 * 8) + (D == 3 ? (ri >> 3) : (ri >> 2));
-    const int Y0 = (D == 3 ? bq1 * 4 : bq1 * 8) + (D == 3 ? ((ri >> 2) & 1) : (ri & 3));
-    const int Zc = D == 3 ? bq2 * 4 + (ri & 3) : 0;
-    // this lane's table columns per sub-block (axes 0 and 1) and on axis 2
-    int cx[4], cy[4];
-#pragma unroll
-    for (int sb = 0; sb < 4; ++sb) {
-        cx[sb] = X0 + SB0 * (sb >> 1);
-        cy[sb] = to1 + Y0 + SB1 * (sb & 1);
-    }
-    const int cz = to2 + Zc;
-    const uint32_t sbm2 = D == 3 ? (0xfu << (bq2 * 4)) : 1u;
+    // the wave's block: 4x4x4 cells in 3-D, 8x8 in 2-D, as bit masks per axis
+    const uint32_t bm0 = D == 3 ? (0xfu << (bq0 * 4)) : (0xffu << (bq0 * 8));
+    const uint32_t bm1 = D == 3 ? (0xfu << (bq1 * 4)) : (0xffu << (bq1 * 8));
+    const uint32_t bm2 = D == 3 ? (0xfu << (bq2 * 4)) : 1u;
     // this lane's table columns and value in the rows of pair 0; a list entry (byte
-    // offset of a pair's rows) added to them addresses the pair's entries
-    const char *px[4], *py[4];
-#pragma unroll
-    for (int sb = 0; sb < 4; ++sb) {
-        px[sb] = (const char *)(s_tab + cx[sb]);
-        py[sb] = (const char *)(s_tab + cy[sb]);
-    }
+    // offset of a pair's rows) added to them addresses the pair's entries.
+    // 3-D: row ri is x-y cell (ri >> 2, ri & 3) of the block, slice s its z plane s:
+    //   qx, qy the row's x and y columns, qz the block's 4 z columns (16-byte aligned:
+    //   the z table starts at T0 + T1, a multiple of 4).
+    // 2-D: row ri is cell (ri >> 3, ri & 7) of a 2 x 8 slice, slice s x rows 2s, 2s + 1:
+    //   qx the row's x column in slice 0 (slice s: + 2s), qy its y column.
     const char *pv = (const char *)(s_val + ri);
-    // 3-D: the lane's x and y table columns (row cell ri of the 4x4x4 block) and the
-    // block's 4 z columns (16-byte aligned: the z table starts at T0 + T1, a multiple of 4)
-    const char *qx = (const char *)(s_tab + bq0 * 4 + (ri >> 2));
-    const char *qy = (const char *)(s_tab + to1 + bq1 * 4 + (ri & 3));
+    const char *qx = (const char *)(s_tab + (D == 3 ? bq0 * 4 + (ri >> 2) : bq0 * 8 + (ri >> 3)));
+    const char *qy = (const char *)(s_tab + to1 + (D == 3 ? bq1 * 4 + (ri & 3) : bq1 * 8 + (ri & 7)));
     const char *qz = (const char *)(s_tab + to2 + bq2 * 4);
 
     if (tid < ttp) s_tab[B * ttp + tid] = 0.f;
@@ -975,67 +961,38 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         f32x4 acc[4];
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) acc[sb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        if constexpr (D == 3) {
-            // phase 3 (3-D): one list of the pairs whose windows reach the wave's 4x4x4
-            // block.  The 16 rows of an MFMA are the block's x-y cells (lane row ri:
-            // x = ri >> 2, y = ri & 3) and the four z slices are four MFMAs: per 4
-            // listed pairs the lane forms its x-y weight once and multiplies it by the
-            // pair's 4 z weights of the block (one 16-byte read).  A pair that misses
-            // a slice adds exact zeros there.
-            const bool hit = (m0 & (0xfu << (bq0 * 4))) && (m1 & (0xfu << (bq1 * 4))) && (m2 & sbm2);
-            const unsigned long long hm = __ballot(hit);
-            const int cnt = __popcll(hm);
-            const int cpad = (cnt + 3) & ~3;
-            int *lst = s_list + wave * 4 * B;
-            const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0));
-            if (hit) lst[pos] = lane << (lgt + 2);
-            if (lane >= cnt && lane < cpad) lst[lane] = B << (lgt + 2);
-            // the list was written by this wave's own lanes: order the LDS accesses
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const int *lst0 = lst + kg;
-            for (int t = 0; t < cpad; t += 4) {
-                const int off = lst0[t]; // byte offset of the pair's rows
+        // phase 3: one list of the pairs whose windows reach the wave's block; per 4
+        // listed pairs a lane forms its weight factors once and feeds the block's four
+        // slice MFMAs (a pair that misses a slice adds exact zeros there)
+        const bool hit = (m0 & bm0) && (m1 & bm1) && (m2 & bm2);
+        const unsigned long long hm = __ballot(hit);
+        const int cnt = __popcll(hm);
+        const int cpad = (cnt + 3) & ~3;
+        int *lst = s_list + wave * B;
+        const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0));
+        if (hit) lst[pos] = lane << (lgt + 2);
+        if (lane >= cnt && lane < cpad) lst[lane] = B << (lgt + 2);
+        // the list was written by this wave's own lanes: order the LDS accesses
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int *lst0 = lst + kg;
+        for (int t = 0; t < cpad; t += 4) {
+            const int off = lst0[t]; // byte offset of the pair's rows
+            const float v = *(const float *)(pv + off);
+            float w[4];
+            if (D == 3) {
                 const float wxy = *(const float *)(qx + off) * *(const float *)(qy + off);
                 const float4 wz = *(const float4 *)(qz + off);
-                const float v = *(const float *)(pv + off);
-                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.x, v, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.y, v, acc[1], 0, 0, 0);
-                acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.z, v, acc[2], 0, 0, 0);
-                acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy * wz.w, v, acc[3], 0, 0, 0);
+                w[0] = wxy * wz.x;
+                w[1] = wxy * wz.y;
+                w[2] = wxy * wz.z;
+                w[3] = wxy * wz.w;
+            } else {
+                const float wy = *(const float *)(qy + off);
+#pragma unroll
+                for (int sl = 0; sl < 4; ++sl) w[sl] = *(const float *)(qx + off + 8 * sl) * wy;
             }
-        } else {
-        // phase 3 (2-D): per 4x4 sub-block, list the pairs that reach it, then 4 pairs
-        // per MFMA
-        int cnt[4], cmax = 0;
 #pragma unroll
-        for (int sb = 0; sb < 4; ++sb) {
-            const uint32_t bm0 = (D == 3 ? 0x3u : 0xfu) << ((D == 3 ? bq0 * 4 : bq0 * 8) + SB0 * (sb >> 1));
-            const uint32_t bm1 = (D == 3 ? 0x3u : 0xfu) << ((D == 3 ? bq1 * 4 : bq1 * 8) + SB1 * (sb & 1));
-            const bool hit = (m0 & bm0) && (m1 & bm1) && (m2 & sbm2);
-            const unsigned long long hm = __ballot(hit);
-            cnt[sb] = __popcll(hm);
-            cmax = max(cmax, (cnt[sb] + 3) & ~3);
-            int *lst = s_list + (wave * 4 + sb) * B;
-            const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0));
-            if (hit) lst[pos] = lane << (lgt + 2);
-        }
-        // every list padded with the zero pair to the longest: the four sub-blocks'
-        // products then run interleaved, their LDS reads in flight together
-#pragma unroll
-        for (int sb = 0; sb < 4; ++sb)
-            if (lane >= cnt[sb] && lane < cmax) s_list[(wave * 4 + sb) * B + lane] = B << (lgt + 2);
-        // the lists were written by this wave's own lanes: order the LDS accesses
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int *lst0 = s_list + wave * 4 * B + kg;
-        for (int t = 0; t < cmax; t += 4) {
-#pragma unroll
-            for (int sb = 0; sb < 4; ++sb) {
-                const int off = lst0[sb * B + t]; // byte offset of the pair's rows
-                const float w = *(const float *)(px[sb] + off) * *(const float *)(py[sb] + off);
-                const float v = *(const float *)(pv + off);
-                acc[sb] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, v, acc[sb], 0, 0, 0);
-            }
-        }
+            for (int sl = 0; sl < 4; ++sl) acc[sl] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[sl], v, acc[sl], 0, 0, 0);
         }
         if (base == item.y) {
             // first batch: the Kahan step from zero sums is exact (S = acc, K = 0)
@@ -1068,15 +1025,13 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     if (!empty && has_block && (ri >> 1) < ncg) {
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
-            // 3-D: sb is the z slice and register v of lane j + 16 kg holds row 4 kg + v,
-            // i.e. x = kg, y = v; 2-D: sb is the 4x4 sub-block, row 4 kg + v its cell
-            const int X = D == 3 ? bq0 * 4 : bq0 * 8 + SB0 * (sb >> 1);
-            const int Y = D == 3 ? bq1 * 4 : bq1 * 8 + SB1 * (sb & 1);
+            // register v of lane j + 16 kg holds row 4 kg + v of slice sb: in 3-D x-y cell
+            // (kg, v) of z plane sb; in 2-D cell ((4 kg + v) >> 3, (4 kg + v) & 7) of x rows
+            // 2 sb, 2 sb + 1, i.e. x = 2 sb + (kg >> 1), y = 4 (kg & 1) + v
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int cell = 4 * kg + v;
-                const int L0 = X + (D == 3 ? kg : (cell >> 2));
-                const int L1 = Y + (D == 3 ? v : (cell & 3));
+                const int L0 = D == 3 ? bq0 * 4 + kg : bq0 * 8 + 2 * sb + (kg >> 1);
+                const int L1 = D == 3 ? bq1 * 4 + v : bq1 * 8 + 4 * (kg & 1) + v;
                 const int L2 = D == 3 ? bq2 * 4 + sb : 0;
                 s_out[(ri >> 1) * CS + ((L0 * T1 + L1) * T2 + L2) * 2 + (ri & 1)] = S[sb][v] - K[sb][v];
             }

@@ -429,41 +429,34 @@ __global__ __launch_bounds__(256) void k_pair_count(int64_t M, const int4 *__res
     cnt[p] = n;
 }
 
-__global__ __launch_bounds__(256) void k_pair_emit(int64_t M, const int4 *__restrict__ rec_cell,
-                                                   const float4 *__restrict__ rec_frac, TileGeom g,
+// The pairs of sample record p are offs[p] .. offs[p] + n - 1 with n = the product of
+// its per-axis tile spans, enumerated leading axis outermost: local index
+// (i * n1 + k) * n2 + m is tile (q0[0] + i, q0[1] + k, q0[2] + m), each wrapped.
+// k_pair_keys writes each pair's spread tile (the sort key) and its record position;
+// after the stable sort by tile, k_pair_records rebuilds every record in sorted order
+// from its sample record, so only 8 bytes per pair are written in sample order.
+__global__ __launch_bounds__(256) void k_pair_keys(int64_t M, const int4 *__restrict__ rec_cell, TileGeom g,
                                                    const int *__restrict__ offs, int *__restrict__ pkey,
-                                                   int4 *__restrict__ pa, int4 *__restrict__ pb)
+                                                   int *__restrict__ psamp)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= M) return;
     const int4 rc = rec_cell[p];
-    const float4 rf = rec_frac[p];
     int q0[3] = {0, 0, 0}, n[3] = {1, 1, 1};
     axis_tile_span(rc.x, g.W, (int)g.nf[0], g.tile[0], q0[0], n[0]);
     axis_tile_span(rc.y, g.W, (int)g.nf[1], g.tile[1], q0[1], n[1]);
     if (g.dim == 3) axis_tile_span(rc.z, g.W, (int)g.nf[2], g.tile[2], q0[2], n[2]);
-    const int4 b = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
     int o = offs[p];
     int t0 = q0[0];
     for (int i = 0; i < n[0]; ++i) {
-        int j0, c0, l0;
-        tap_box(rc.x, t0, g.tile[0], (int)g.nf[0], g.W, j0, c0, l0);
         int t1 = q0[1];
         for (int k = 0; k < n[1]; ++k) {
-            int j1, c1, l1;
-            tap_box(rc.y, t1, g.tile[1], (int)g.nf[1], g.W, j1, c1, l1);
             int t2 = q0[2];
             for (int m = 0; m < n[2]; ++m) {
-                int j2 = 0, c2 = 1, l2 = 0;
                 int key = t0 * g.ntiles[1] + t1;
-                if (g.dim == 3) {
-                    tap_box(rc.z, t2, g.tile[2], (int)g.nf[2], g.W, j2, c2, l2);
-                    key = key * g.ntiles[2] + t2;
-                }
+                if (g.dim == 3) key = key * g.ntiles[2] + t2;
                 pkey[o] = key;
-                pa[o] = make_int4(rc.w, j0 | (j1 << 8) | (j2 << 16), c0 | (c1 << 8) | (c2 << 16),
-                                  l0 | (l1 << 10) | (l2 << 20));
-                pb[o] = b;
+                psamp[o] = (int)p;
                 ++o;
                 if (++t2 == g.ntiles[2]) t2 = 0;
             }
@@ -473,16 +466,40 @@ __global__ __launch_bounds__(256) void k_pair_emit(int64_t M, const int4 *__rest
     }
 }
 
-__global__ __launch_bounds__(256) void k_permute_pairs(int64_t n, const int *__restrict__ sorted,
-                                                       const int4 *__restrict__ ia, const int4 *__restrict__ ib,
-                                                       int4 *__restrict__ oa, int4 *__restrict__ ob)
+// Pair q of the tile-sorted order: original pair index o = sorted[q], its sample record
+// p = psamp[o] and local index o - offs[p] give the tile on each axis; the record is
+//   a = {orig, j0 | j1<<8 | j2<<16, c0 | c1<<8 | c2<<16, l0 | l1<<10 | l2<<20}
+//   b = {t0, t1, t2, dcf} (float bits)
+// (j = first window tap inside the tile, c = taps inside, l = tile-local cell of tap j).
+__global__ __launch_bounds__(256) void k_pair_records(int64_t P, const int *__restrict__ sorted,
+                                                      const int *__restrict__ psamp, const int *__restrict__ offs,
+                                                      const int4 *__restrict__ rec_cell,
+                                                      const float4 *__restrict__ rec_frac, TileGeom g,
+                                                      int4 *__restrict__ pa, int4 *__restrict__ pb)
 {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) {
-        const int i = sorted[q];
-        oa[q] = ia[i];
-        ob[q] = ib[i];
-    }
+    if (q >= P) return;
+    const int o = sorted[q];
+    const int p = psamp[o];
+    const int4 rc = rec_cell[p];
+    const float4 rf = rec_frac[p];
+    int q0[3] = {0, 0, 0}, n[3] = {1, 1, 1};
+    axis_tile_span(rc.x, g.W, (int)g.nf[0], g.tile[0], q0[0], n[0]);
+    axis_tile_span(rc.y, g.W, (int)g.nf[1], g.tile[1], q0[1], n[1]);
+    if (g.dim == 3) axis_tile_span(rc.z, g.W, (int)g.nf[2], g.tile[2], q0[2], n[2]);
+    const int local = o - offs[p];
+    const int m = local % n[2], r = local / n[2];
+    const int k = r % n[1], i = r / n[1];
+    int t0 = q0[0] + i, t1 = q0[1] + k, t2 = q0[2] + m;
+    if (t0 >= g.ntiles[0]) t0 -= g.ntiles[0];
+    if (t1 >= g.ntiles[1]) t1 -= g.ntiles[1];
+    if (t2 >= g.ntiles[2]) t2 -= g.ntiles[2];
+    int j0, c0, l0, j1, c1, l1, j2 = 0, c2 = 1, l2 = 0;
+    tap_box(rc.x, t0, g.tile[0], (int)g.nf[0], g.W, j0, c0, l0);
+    tap_box(rc.y, t1, g.tile[1], (int)g.nf[1], g.W, j1, c1, l1);
+    if (g.dim == 3) tap_box(rc.z, t2, g.tile[2], (int)g.nf[2], g.W, j2, c2, l2);
+    pa[q] = make_int4(rc.w, j0 | (j1 << 8) | (j2 << 16), c0 | (c1 << 8) | (c2 << 16), l0 | (l1 << 10) | (l2 << 20));
+    pb[q] = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
 }
 
 // ---------------------------------------------------------------------------
@@ -2105,7 +2122,7 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
                            frac, dcf, R.cell, R.frac);
         GN_HIP(hipGetLastError());
 
-        // (spread tile, sample) pair records, generated in record order
+        // (spread tile, sample) pairs: keys in record order, sorted by tile, records rebuilt in tile order
         TileGeom tg;
         tg.dim = g.dim;
         for (int a = 0; a < 3; ++a) {
@@ -2130,14 +2147,12 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         if (hv[1]) return fail(GN_EINVAL, "non-finite sample coordinate");
         const int64_t P = (int64_t)(unsigned)hv[0];
         if (P > (int64_t)INT32_MAX) return fail(GN_EINVAL, "too many (tile, sample) pairs");
-        int *pkey = nullptr, *psorted = nullptr, *pstart = nullptr;
-        int4 *ta = nullptr, *tb = nullptr;
-        if ((rc = sc.alloc(&pkey, (size_t)P, "pairs")) || (rc = sc.alloc(&ta, (size_t)P, "pairs")) ||
-            (rc = sc.alloc(&tb, (size_t)P, "pairs")) || (rc = sc.alloc(&psorted, (size_t)P, "pairs")) ||
-            (rc = sc.alloc(&pstart, (size_t)K + 1, "pairs")))
+        int *pkey = nullptr, *psamp = nullptr, *psorted = nullptr, *pstart = nullptr;
+        if ((rc = sc.alloc(&pkey, (size_t)P, "pairs")) || (rc = sc.alloc(&psamp, (size_t)P, "pairs")) ||
+            (rc = sc.alloc(&psorted, (size_t)P, "pairs")) || (rc = sc.alloc(&pstart, (size_t)K + 1, "pairs")))
             return rc;
-        hipLaunchKernelGGL(k_pair_emit, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, R.cell, R.frac, tg,
-                           pcnt, pkey, ta, tb);
+        hipLaunchKernelGGL(k_pair_keys, dim3(grid_blocks(M, 256, INT32_MAX)), dim3(256), 0, st, M, R.cell, tg, pcnt,
+                           pkey, psamp);
         GN_HIP(hipGetLastError());
         if ((rc = stable_sort_by_key(P, pkey, 1, K, psorted, pstart, st, sc))) return rc;
         if (P > R.pairs_cap) {
@@ -2145,8 +2160,8 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
                 return rc;
             R.pairs_cap = P;
         }
-        hipLaunchKernelGGL(k_permute_pairs, dim3(grid_blocks(P, 256, INT32_MAX)), dim3(256), 0, st, P, psorted, ta, tb,
-                           R.pair_a, R.pair_b);
+        hipLaunchKernelGGL(k_pair_records, dim3(grid_blocks(P, 256, INT32_MAX)), dim3(256), 0, st, P, psorted, psamp,
+                           pcnt, R.cell, R.frac, tg, R.pair_a, R.pair_b);
         GN_HIP(hipGetLastError());
         GN_HIP(hipMemcpyAsync(h_home.data(), R.bin_start, sizeof(int) * (Kg + 1), hipMemcpyDeviceToHost, st));
         GN_HIP(hipMemcpyAsync(h_pair.data(), pstart, sizeof(int) * (K + 1), hipMemcpyDeviceToHost, st));

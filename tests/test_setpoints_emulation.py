@@ -3,7 +3,7 @@
 Restates, in NumPy, what the set_points kernels of mripy_amd/csrc/gn_kernels.hip
 compute for every sample -- k_prep (FP64 window start, offset, home gather tile),
 the digit-wise stable counting sort (k_radix_count / k_radix_scatter /
-k_bin_starts_sorted), k_pair_count / k_pair_emit (axis_tiles, tap_box: the spread
+k_bin_starts_sorted), k_pair_count / k_pair_keys / k_pair_records (axis_tiles, tap_box: the spread
 tiles a window touches, the taps inside each, packed into 8- and 10-bit fields),
 k_bank_order (LDS-bank permutation of each gather work item) -- and the LDS
 addresses the gather and trajectory-gradient tap loops form from them, and asserts
@@ -168,7 +168,11 @@ def emulate(coords, modes, ns, rat=2, check_pairs=True):
             q0, na = axis_tile_span(int(cells[i, a]), W, nf[a], tile[a])
             assert tl[a] == [(q0 + k) % ntiles[a] for k in range(na)]
         n = int(np.prod([len(x) for x in tl]))
-        for combo in np.ndindex(*[len(x) for x in tl]):
+        ns_ax = [len(x) for x in tl] + [1] * (3 - dim)
+        for local, combo in enumerate(np.ndindex(*[len(x) for x in tl])):
+            # k_pair_records: local index (i * n1 + k) * n2 + m of the emission order
+            m_, r_ = local % ns_ax[2], local // ns_ax[2]
+            assert (r_ // ns_ax[1], r_ % ns_ax[1], m_)[:dim] == tuple(combo)
             kk = 0
             for a in range(dim):
                 q = tl[a][combo[a]]

@@ -829,6 +829,8 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     const int wave = tid >> 6;
     const int lane = tid & 63;
     const int nwaves = blockDim.x >> 6;
+    const bool split = nwaves >= 2; // phase 1 spread over two waves
+    static_assert(kSpreadBatch <= 64, "a batch's pairs are one wave's lanes");
     const int cg0 = cgi * NCG;
     const int ncg = (int)min((int64_t)NCG, a.nc - cg0);
     const int T0 = a.tile[0], T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
@@ -890,13 +892,19 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     for (int base = item.y; base < item.z; base += B) {
         const int n = min(B, item.z - base);
         __syncthreads(); // previous batch fully consumed
-        // phase 1: one thread per pair loads its record and coil values, builds the masks
-        if (tid < n) {
-            const int q = base + tid;
+        // phase 1: per pair, a thread of the first wave loads the coil values and a
+        // thread of the second wave (the same thread with one wave) builds the masks and
+        // window parameters; each loads the pair's record
+        const int pt = split ? (tid & 63) : tid;
+        const bool do_vals = tid < n;
+        const bool do_msk = split ? (tid >= 64 && pt < n) : tid < n;
+        if (do_vals || do_msk) {
+            const int q = base + pt;
             const int4 pa = a.pair_a[q];
             const int4 pbi = a.pair_b[q];
             const float4 pb = make_float4(__int_as_float(pbi.x), __int_as_float(pbi.y), __int_as_float(pbi.z),
                                           __int_as_float(pbi.w));
+            if (do_vals) {
             const float scale = a.use_dcf ? pb.w : 1.0f;
             float2 v[NCG];
             // coil-last with an even coil count: the pair's coils are contiguous 16-byte pairs
@@ -921,7 +929,7 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
                     }
                 }
             }
-            float4 *dv = (float4 *)(s_val + tid * ttp);
+            float4 *dv = (float4 *)(s_val + pt * ttp);
 #pragma unroll
             for (int c2 = 0; c2 < 4; ++c2) {
                 float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -932,6 +940,8 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
                 }
                 dv[c2] = o;
             }
+            }
+            if (do_msk) {
             // per axis: the window's tile-local cells (L - l) mod T < c as a bit mask,
             // and the parameters of its weights
             const int Ts[3] = {T0, T1, T2};
@@ -943,8 +953,9 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
                 const int j = (pa.y >> (8 * ax)) & 0xff;
                 uint64_t m = ((1ull << c) - 1) << l;
                 m = (m | (m >> Ts[ax])) & ((1ull << Ts[ax]) - 1);
-                s_msk[tid * 4 + ax] = (int)(uint32_t)m;
-                s_par[tid * 4 + ax] = make_int4(l, c, a.ns - j, __float_as_int(ts[ax]));
+                s_msk[pt * 4 + ax] = (int)(uint32_t)m;
+                s_par[pt * 4 + ax] = make_int4(l, c, a.ns - j, __float_as_int(ts[ax]));
+            }
             }
         }
         __syncthreads();

@@ -117,6 +117,17 @@ struct WorkItems {
     int mask_k = 1;
 };
 
+// Gather work items {tile, sbeg, send, 0}: each gather tile's samples (bin offsets
+// h_home[Kg + 1]) in slices of at most chunk, in tile order.
+inline void build_gather_items(int64_t Kg, const int *h_home, int chunk, std::vector<Item> &items)
+{
+    items.clear();
+    for (int64_t k = 0; k < Kg; ++k) {
+        const int hb = h_home[k], he = h_home[k + 1];
+        for (int s0 = hb; s0 < he; s0 += chunk) items.push_back({(int)k, s0, std::min(he, s0 + chunk), 0});
+    }
+}
+
 // Work items from the bin offsets: h_pair[K + 1] (pairs of each spread tile) and
 // h_home[Kg + 1] (samples of each gather tile).  Every spread tile gets at least one
 // item; tiles with more than spread_chunk pairs are split into slices written to
@@ -139,10 +150,7 @@ inline void build_work_items(int64_t K, int64_t Kg, const int *ntiles, const int
             w.slots += nch;
         }
     }
-    for (int64_t k = 0; k < Kg; ++k) {
-        const int hb = h_home[k], he = h_home[k + 1];
-        for (int s0 = hb; s0 < he; s0 += gather_chunk) w.gather.push_back({(int)k, s0, std::min(he, s0 + gather_chunk), 0});
-    }
+    build_gather_items(Kg, h_home, gather_chunk, w.gather);
     // one mask word per column of spread tiles; bit b: leading-axis tiles [b k, (b + 1) k)
     const int64_t ncolt = (int64_t)ntiles[1] * ntiles[2];
     w.mask_k = (ntiles[0] + 63) / 64;

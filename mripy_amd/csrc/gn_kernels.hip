@@ -502,6 +502,60 @@ __global__ __launch_bounds__(256) void k_pair_records(int64_t P, const int *__re
     pb[q] = make_int4(__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w));
 }
 
+// Spread work items, built on the device from the tile offsets pstart[K + 1] (the
+// rule of the host planner's build_work_items): tile k with n pairs gets one item
+// {k, pbeg, pend, -1}, or, when n > chunk, ceil(n / chunk) slices {k, ..., slot} that
+// write consecutive partial slots and one reduce item {k, slot0, slices, 0}.  Items,
+// slots and reduce items are numbered in tile order by exclusive scans of the
+// per-tile counts that k_item_counts writes.
+__global__ __launch_bounds__(256) void k_item_counts(int64_t K, const int *__restrict__ pstart, int chunk,
+                                                     int *__restrict__ nitems, int *__restrict__ nslots,
+                                                     int *__restrict__ nsplit)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int n = pstart[k + 1] - pstart[k];
+    const int nch = (n + chunk - 1) / chunk;
+    const bool split = n > chunk;
+    nitems[k] = split ? nch : 1;
+    nslots[k] = split ? nch : 0;
+    nsplit[k] = split ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_item_write(int64_t K, const int *__restrict__ pstart, int chunk,
+                                                    const int *__restrict__ ioff, const int *__restrict__ soff,
+                                                    const int *__restrict__ roff, int4 *__restrict__ items,
+                                                    int4 *__restrict__ reduce)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int b = pstart[k], e = pstart[k + 1];
+    const int o = ioff[k];
+    if (e - b <= chunk) {
+        items[o] = make_int4((int)k, b, e, -1);
+        return;
+    }
+    const int nch = (e - b + chunk - 1) / chunk, s0 = soff[k];
+    reduce[roff[k]] = make_int4((int)k, s0, nch, 0);
+    for (int c = 0; c < nch; ++c) items[o + c] = make_int4((int)k, b + c * chunk, min(e, b + (c + 1) * chunk), s0 + c);
+}
+
+// Occupancy row masks: one word per column of spread tiles (ncolt = ntiles[1] *
+// ntiles[2] columns), bit b set when one of the leading-axis tiles [b mask_k,
+// (b + 1) mask_k) of the column holds pairs; one thread per column, no atomics.
+__global__ __launch_bounds__(256) void k_occ_masks(int64_t ncolt, int nt0, int mask_k, const int *__restrict__ pstart,
+                                                   unsigned long long *__restrict__ occ)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncolt) return;
+    unsigned long long m = 0ull;
+    for (int t0 = 0; t0 < nt0; ++t0) {
+        const int64_t k = t0 * ncolt + c;
+        if (pstart[k + 1] > pstart[k]) m |= 1ull << (t0 / mask_k);
+    }
+    occ[c] = m;
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint spread: output-stationary tiles accumulated in registers.
 //
@@ -2098,7 +2152,9 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
     R.npairs = 0;
 
     Scratch sc(st);
-    std::vector<int> h_home(Kg + 1, 0), h_pair(K + 1, 0);
+    std::vector<int> h_home(Kg + 1, 0);
+    int counts[3] = {0, 0, 0}; // spread items, partial slots, reduce items
+    int *tile_pairs = nullptr; // K + 1 offsets of each spread tile's pairs (scratch)
 
     if (M > 0) {
         int4 *cell = nullptr;
@@ -2174,37 +2230,63 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         hipLaunchKernelGGL(k_pair_records, dim3(grid_blocks(P, 256, INT32_MAX)), dim3(256), 0, st, P, psorted, psamp,
                            pcnt, R.cell, R.frac, tg, R.pair_a, R.pair_b);
         GN_HIP(hipGetLastError());
-        GN_HIP(hipMemcpyAsync(h_home.data(), R.bin_start, sizeof(int) * (Kg + 1), hipMemcpyDeviceToHost, st));
-        GN_HIP(hipMemcpyAsync(h_pair.data(), pstart, sizeof(int) * (K + 1), hipMemcpyDeviceToHost, st));
-        GN_HIP(hipStreamSynchronize(st));
+        tile_pairs = pstart;
         R.npairs = P;
     } else {
         GN_HIP(hipMemsetAsync(R.bin_start, 0, sizeof(int) * (Kg + 1), st));
+        if ((rc = sc.alloc(&tile_pairs, (size_t)K + 1, "pairs"))) return rc;
+        GN_HIP(hipMemsetAsync(tile_pairs, 0, sizeof(int) * (K + 1), st));
+    }
+
+    // spread work items, partial slots, reduce items and occupancy masks on the device;
+    // their counts come back with the gather bin offsets in one synchronisation
+    {
+        const int64_t P = R.npairs;
+        const int64_t cap_items = K + P / kSpreadChunk + 1, cap_red = std::min<int64_t>(K, P / kSpreadChunk + 1);
+        const int64_t ncolt = (int64_t)g.ntiles[1] * g.ntiles[2];
+        if ((rc = grow_alloc(&R.spread_items, &R.spread_items_cap, (size_t)cap_items, "work items")) ||
+            (rc = grow_alloc(&R.reduce_items, &R.reduce_items_cap, (size_t)cap_red, "work items")) ||
+            (rc = grow_alloc(&R.occ_mask, &R.occ_mask_cap, (size_t)ncolt, "tile masks")))
+            return rc;
+        const int64_t nbk = (K + kScanTile - 1) / kScanTile;
+        int *ni = nullptr, *ns = nullptr, *nr = nullptr, *bi = nullptr, *bs = nullptr, *br = nullptr;
+        if ((rc = sc.alloc(&ni, (size_t)K, "work items")) || (rc = sc.alloc(&ns, (size_t)K, "work items")) ||
+            (rc = sc.alloc(&nr, (size_t)K, "work items")) || (rc = sc.alloc(&bi, (size_t)nbk + 1, "work items")) ||
+            (rc = sc.alloc(&bs, (size_t)nbk + 1, "work items")) || (rc = sc.alloc(&br, (size_t)nbk + 1, "work items")))
+            return rc;
+        const dim3 gk(grid_blocks(K, 256, INT32_MAX));
+        hipLaunchKernelGGL(k_item_counts, gk, dim3(256), 0, st, K, tile_pairs, kSpreadChunk, ni, ns, nr);
+        GN_HIP(hipGetLastError());
+        GN_HIP(device_scan(ni, K, bi, st));
+        GN_HIP(device_scan(ns, K, bs, st));
+        GN_HIP(device_scan(nr, K, br, st));
+        hipLaunchKernelGGL(k_item_write, gk, dim3(256), 0, st, K, tile_pairs, kSpreadChunk, ni, ns, nr, R.spread_items,
+                           R.reduce_items);
+        GN_HIP(hipGetLastError());
+        R.mask_k = (g.ntiles[0] + 63) / 64;
+        hipLaunchKernelGGL(k_occ_masks, dim3(grid_blocks(ncolt, 256, INT32_MAX)), dim3(256), 0, st, ncolt, g.ntiles[0],
+                           R.mask_k, tile_pairs, R.occ_mask);
+        GN_HIP(hipGetLastError());
+        GN_HIP(hipMemcpyAsync(&counts[0], bi + nbk, sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipMemcpyAsync(&counts[1], bs + nbk, sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipMemcpyAsync(&counts[2], br + nbk, sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipMemcpyAsync(h_home.data(), R.bin_start, sizeof(int) * (Kg + 1), hipMemcpyDeviceToHost, st));
         GN_HIP(hipStreamSynchronize(st));
     }
 
     // work items: every spread tile gets at least one item (empty tiles write zeros),
     // and the occupancy row masks for the first adjoint FFT pass (the spread leaves
     // tiles without pairs unwritten where that pass skips them)
-    host::WorkItems wi;
-    host::build_work_items(K, Kg, g.ntiles, h_pair.data(), h_home.data(), kSpreadChunk, kGatherChunk, wi);
-    if ((rc = grow_alloc(&R.spread_items, &R.spread_items_cap, wi.spread.size(), "work items")) ||
-        (rc = grow_alloc(&R.reduce_items, &R.reduce_items_cap, wi.reduce.size(), "work items")) ||
-        (rc = grow_alloc(&R.gather_items, &R.gather_items_cap, wi.gather.size(), "work items")) ||
-        (rc = grow_alloc(&R.occ_mask, &R.occ_mask_cap, wi.occ.size(), "tile masks")))
-        return rc;
-    GN_HIP(hipMemcpy(R.spread_items, wi.spread.data(), sizeof(int4) * wi.spread.size(), hipMemcpyHostToDevice));
-    if (!wi.reduce.empty())
-        GN_HIP(hipMemcpy(R.reduce_items, wi.reduce.data(), sizeof(int4) * wi.reduce.size(), hipMemcpyHostToDevice));
-    if (!wi.gather.empty())
-        GN_HIP(hipMemcpy(R.gather_items, wi.gather.data(), sizeof(int4) * wi.gather.size(), hipMemcpyHostToDevice));
-    GN_HIP(hipMemcpy(R.occ_mask, wi.occ.data(), sizeof(unsigned long long) * wi.occ.size(), hipMemcpyHostToDevice));
-    R.n_spread_items = (int)wi.spread.size();
-    R.n_reduce_items = (int)wi.reduce.size();
-    R.n_gather_items = (int)wi.gather.size();
-    R.n_slots = wi.slots;
-    R.mask_k = wi.mask_k;
-    const int slots = wi.slots;
+    std::vector<host::Item> gitems;
+    host::build_gather_items(Kg, h_home.data(), kGatherChunk, gitems);
+    if ((rc = grow_alloc(&R.gather_items, &R.gather_items_cap, gitems.size(), "work items"))) return rc;
+    if (!gitems.empty())
+        GN_HIP(hipMemcpy(R.gather_items, gitems.data(), sizeof(int4) * gitems.size(), hipMemcpyHostToDevice));
+    R.n_spread_items = counts[0];
+    R.n_slots = counts[1];
+    R.n_reduce_items = counts[2];
+    R.n_gather_items = (int)gitems.size();
+    const int slots = counts[1];
     if (R.n_gather_items > 0) {
         const GatherArgs ga = make_gather_args(p, nullptr);
         if (g.dim == 3)

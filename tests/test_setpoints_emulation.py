@@ -305,3 +305,74 @@ def test_row_staging_covers_region_once(modes, ns, rat):
             lin = int(np.ravel_multi_index(r, region))
             want[lin] = int(np.ravel_multi_index(q, nf))
         assert all(want[cidx] == off for cidx, off in w)
+
+
+def spread_items_host(pstart, chunk):
+    """build_work_items (gn_host_plan.h), spread part: items, reduce items, slots."""
+    items, red, slots = [], [], 0
+    for k in range(len(pstart) - 1):
+        b, e = int(pstart[k]), int(pstart[k + 1])
+        if e - b <= chunk:
+            items.append((k, b, e, -1))
+        else:
+            nch = -(-(e - b) // chunk)
+            red.append((k, slots, nch, 0))
+            for c in range(nch):
+                items.append((k, b + c * chunk, min(e, b + (c + 1) * chunk), slots + c))
+            slots += nch
+    return items, red, slots
+
+
+def spread_items_device(pstart, chunk):
+    """k_item_counts, three exclusive scans and k_item_write (gn_kernels.hip)."""
+    K = len(pstart) - 1
+    n = np.diff(pstart)
+    nch = -(-n // chunk)
+    split = n > chunk
+    ni = np.where(split, nch, 1)
+    ns = np.where(split, nch, 0)
+    nr = split.astype(np.int64)
+    ioff = np.concatenate([[0], np.cumsum(ni)])
+    soff = np.concatenate([[0], np.cumsum(ns)])
+    roff = np.concatenate([[0], np.cumsum(nr)])
+    items = [None] * int(ioff[-1])
+    red = [None] * int(roff[-1])
+    for k in range(K):
+        b, e = int(pstart[k]), int(pstart[k + 1])
+        o = int(ioff[k])
+        if e - b <= chunk:
+            items[o] = (k, b, e, -1)
+            continue
+        c_n, s0 = int(nch[k]), int(soff[k])
+        red[int(roff[k])] = (k, s0, c_n, 0)
+        for c in range(c_n):
+            items[o + c] = (k, b + c * chunk, min(e, b + (c + 1) * chunk), s0 + c)
+    return items, red, int(soff[-1])
+
+
+@pytest.mark.parametrize("seed,K,P", [(0, 1, 0), (1, 7, 3000), (2, 4096, 70000), (3, 300, 0)])
+def test_device_spread_items_follow_host_rule(seed, K, P):
+    """The device build of the spread work items (per-tile counts, exclusive scans,
+    writes) gives exactly the host planner's items, reduce items and slot count, for
+    empty, uniform and clustered tile loads; and the occupancy masks (k_occ_masks)
+    mark exactly the columns' tiles that hold pairs."""
+    rng = np.random.default_rng(seed)
+    cnt = np.zeros(K, np.int64)
+    if P:
+        k = np.where(rng.random(P) < 0.5, rng.integers(0, K, P), rng.integers(0, max(1, K // 50), P))
+        np.add.at(cnt, k, 1)
+    pstart = np.concatenate([[0], np.cumsum(cnt)])
+    assert spread_items_device(pstart, 512) == spread_items_host(pstart, 512)
+    nt = [16, 16, K // 256] if K % 256 == 0 else [K, 1, 1]
+    ncolt = nt[1] * nt[2]
+    mask_k = (nt[0] + 63) // 64
+    for c in range(ncolt):
+        m = 0
+        for t0 in range(nt[0]):
+            if pstart[t0 * ncolt + c + 1] > pstart[t0 * ncolt + c]:
+                m |= 1 << (t0 // mask_k)
+        want = 0
+        for it in spread_items_host(pstart, 512)[0]:
+            if it[2] > it[1] and it[0] % ncolt == c:
+                want |= 1 << ((it[0] // ncolt) // mask_k)
+        assert m == want

@@ -568,10 +568,19 @@ int pick_cb(int L)
     return cb;
 }
 
+// Coils per row-pass block: at most 16, fewer while the block's LDS exceeds this cap.
+// The row passes are latency-bound per block, so a small cap (more, smaller blocks per
+// CU) pays: on the 64-instance 2-D workload (lines of 256-640 points, up to 32 coils)
+// caps of 64, 32, 24, 20, 16 KiB gave 8.76, 8.03, 7.92, 7.95, 8.36 ms per batched
+// fwd+adj; 24 KiB keeps the 8 coils of a 256-point 3-D line in one block.
+#ifndef GN_FFT_ROW_LDS_MAX
+#define GN_FFT_ROW_LDS_MAX (24 * 1024)
+#endif
+
 int pick_cl(int L, int64_t nc)
 {
     int cl = (int)std::min<int64_t>(nc, 16);
-    while (cl > 1 && row_lds(L, cl) > 64 * 1024) --cl;
+    while (cl > 1 && row_lds(L, cl) > GN_FFT_ROW_LDS_MAX) --cl;
     return cl;
 }
 

@@ -57,18 +57,21 @@ __device__ inline float2 mul_si(float2 a, float s) { return make_float2(-s * a.y
 
 // One radix-r butterfly on LDS elements at idx0 + k*step_q (k < r) of a line, after
 // twiddling element k by w_g^{t k}: tw[(t k L / g)] (the table in LDS already holds
-// the twiddles of this transform's sign).
-template <int R>
-__device__ inline void butterfly(float2 *buf, int base, int qstride, const float2 *tw, int twstep, int t, float sign)
+// the twiddles of this transform's sign).  TW = false for the first stage (g = r, all
+// twiddles 1).  Otherwise every element is multiplied, t = 0 included (tw[0] = 1 and
+// x * 1 - y * 0 = x exactly), so the lanes of a wave take no per-element branches.
+template <int R, bool TW>
+__device__ __forceinline__ void butterfly(float2 *buf, int base, int qstride, const float2 *tw, int twstep, int t, float sign)
 {
     float2 v[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        float2 a = buf[base + k * qstride];
-        if (k > 0 && t > 0) {
-            a = cmul(a, tw[t * k * twstep]);
-        }
-        v[k] = a;
+    for (int k = 0; k < R; ++k) v[k] = buf[base + k * qstride];
+    if (TW) {
+        float2 w[R];
+#pragma unroll
+        for (int k = 1; k < R; ++k) w[k] = tw[t * k * twstep];
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], w[k]);
     }
     if (R == 2) {
         buf[base] = cadd(v[0], v[1]);
@@ -140,37 +143,47 @@ __device__ inline int fdiv(int n, int d, float r)
 
 // In-place FFT of `nlines` lines of length L held in LDS: element i of line j at
 // buf[j * lstride + i * estride].  All threads of the block take part.
-__device__ inline void fft_lines(float2 *buf, int nlines, int lstride, int estride, int L, const Stages &st,
+__device__ __forceinline__ void fft_lines(float2 *buf, int nlines, int lstride, int estride, int L, const Stages &st,
                                  const float2 *tw, float sign)
 {
-    const float rl = 1.0f / (float)nlines;
+    // fdiv corrects its quotient estimate, so approximate reciprocals suffice
+    const float rl = __builtin_amdgcn_rcpf((float)nlines);
     // power-of-two line counts and butterfly groups index with shifts
     const int lgl = (nlines & (nlines - 1)) == 0 ? __builtin_ctz(nlines) : -1;
-    int q = 1;
     for (int s = 0; s < st.n; ++s) {
         const int r = st.radix[s];
+        const int q = st.q[s];         // butterfly span before this stage
         const int g = r * q;
-        const int twstep = L / g;
-        const int nb = L / r; // butterflies per line
-        const float rq = 1.0f / (float)q;
+        const int twstep = st.tws[s];  // L / g
+        const int nbl = st.nb[s] * nlines; // butterflies of all lines
+        const float rq = __builtin_amdgcn_rcpf((float)q);
         const int lgq = (q & (q - 1)) == 0 ? __builtin_ctz(q) : -1;
-        for (int w = threadIdx.x; w < nb * nlines; w += blockDim.x) {
+        for (int w = threadIdx.x; w < nbl; w += blockDim.x) {
             // lines fastest: neighbouring threads work on neighbouring lines
             const int bidx = lgl >= 0 ? (w >> lgl) : fdiv(w, nlines, rl);
             const int j = w - bidx * nlines;
             const int blk = lgq >= 0 ? (bidx >> lgq) : fdiv(bidx, q, rq);
             const int t = bidx - blk * q;
             const int base = j * lstride + (blk * g + t) * estride;
-            switch (r) {
-            case 2: butterfly<2>(buf, base, q * estride, tw, twstep, t, sign); break;
-            case 3: butterfly<3>(buf, base, q * estride, tw, twstep, t, sign); break;
-            case 4: butterfly<4>(buf, base, q * estride, tw, twstep, t, sign); break;
-            case 8: butterfly<8>(buf, base, q * estride, tw, twstep, t, sign); break;
-            default: butterfly<5>(buf, base, q * estride, tw, twstep, t, sign); break;
+            if (s == 0) {
+                switch (r) {
+                case 2: butterfly<2, false>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 3: butterfly<3, false>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 4: butterfly<4, false>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 8: butterfly<8, false>(buf, base, q * estride, tw, twstep, t, sign); break;
+                default: butterfly<5, false>(buf, base, q * estride, tw, twstep, t, sign); break;
+                }
+            } else {
+                switch (r) {
+                case 2: butterfly<2, true>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 3: butterfly<3, true>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 4: butterfly<4, true>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 8: butterfly<8, true>(buf, base, q * estride, tw, twstep, t, sign); break;
+                default: butterfly<5, true>(buf, base, q * estride, tw, twstep, t, sign); break;
+                }
             }
         }
         __syncthreads();
-        q = g;
     }
 }
 
@@ -552,7 +565,14 @@ Stages make_stages(const FFTAxis &ax)
 {
     Stages s;
     s.n = ax.nst;
-    for (int i = 0; i < kFFTMaxStages; ++i) s.radix[i] = i < ax.nst ? ax.radix[i] : 1;
+    int q = 1;
+    for (int i = 0; i < kFFTMaxStages; ++i) {
+        s.radix[i] = i < ax.nst ? ax.radix[i] : 1;
+        s.q[i] = q;
+        s.tws[i] = ax.L / (q * s.radix[i]);
+        s.nb[i] = ax.L / s.radix[i];
+        q *= s.radix[i];
+    }
     return s;
 }
 

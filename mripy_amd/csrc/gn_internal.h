@@ -43,9 +43,16 @@ struct FFTAxis {
 };
 
 // launch arguments of the pruned FFT passes (fft.hip)
+
+// Radix stages of a length-L transform and, per stage, the butterfly span q before it
+// (product of the earlier radices), the twiddle step L / (q r) and the butterflies per
+// line L / r.
 struct Stages {
     int n;
     int radix[kFFTMaxStages];
+    int q[kFFTMaxStages];
+    int tws[kFFTMaxStages];
+    int nb[kFFTMaxStages];
 };
 
 struct ColArgs {

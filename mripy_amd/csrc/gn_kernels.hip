@@ -841,10 +841,10 @@ __global__ __launch_bounds__(64 * kSpreadWavesMax) void k_spread_batch(const Spr
 // so the result is deterministic; batch sums are folded into Kahan-compensated
 // registers as in spread_body.  Lane layout of the 16x16x4 f32 operation:
 // A[i][k] and B[k][j] in lane i + 16k and j + 16k, D[4g + v][j] in register v of
-// lane j + 16g.  The finished tile goes through LDS ([coil][cell]) and is written
-// row by row along the last axis.  Needs block-aligned tiles (multiples of 4 per
-// axis in 3-D, of 8 in 2-D) of at most kSpreadWavesMax blocks: one workgroup per
-// tile and coil group.
+// lane j + 16g.  Each lane writes its cells of the finished tile straight from its
+// registers (16-byte stores of two cells along the last axis).  Needs block-aligned
+// tiles (multiples of 4 per axis in 3-D, of 8 in 2-D) of at most kSpreadWavesMax
+// blocks: one workgroup per tile and coil group.
 // ---------------------------------------------------------------------------
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -852,10 +852,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 inline size_t spread_mfma_lds_bytes(const SpreadArgs &a, int waves)
 {
     const int ttp = a.TT <= 16 ? 16 : a.TT <= 32 ? 32 : a.TT <= 64 ? 64 : 128; // table row stride
-    const size_t batch = (size_t)(kSpreadBatch + 1) * ttp * 4 * 2 + (size_t)kSpreadBatch * 16 +
-                         (size_t)kSpreadBatch * 64 + (size_t)waves * kSpreadBatch * 4;
-    const size_t out = (size_t)8 * (a.TC + 1) * 8; // output staging: up to 8 coils of the tile
-    return std::max(batch, out);
+    return (size_t)(kSpreadBatch + 1) * ttp * 4 * 2 + (size_t)kSpreadBatch * 16 + (size_t)kSpreadBatch * 64 +
+           (size_t)waves * kSpreadBatch * 4;
 }
 
 template <int D, int NCG>
@@ -1094,32 +1092,17 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
                 }
         }
     }
-    // write through LDS: every wave stores its sub-blocks' sums as [coil][cell]
-    // (a cell of padding per coil), then the workgroup writes the tile row by row
-    // along the last axis, two cells per 16-byte store where alignment allows
-    // an item without pairs (a tile no window reaches) writes zeros, skipping the
-    // staging; the condition is the same for the whole workgroup
-    const bool empty = item.y >= item.z;
-    float *s_out = lds_sp;
+    // Write-out straight from the registers.  Lane (ri, kg) holds re (ri even) or im
+    // (ri odd) of coil ri >> 1 for 16 cells; for each of 4 outer steps i its 4 values
+    // o[z] are consecutive cells along the last axis (3-D: z planes sb of x-y cell
+    // (kg, i); 2-D: y cells v of slice i).  Neighbouring lanes swap halves (DPP) so
+    // that the even lane holds cells z = 0, 1 and the odd lane cells z = 2, 3 as
+    // (re, im) pairs: one 16-byte store per lane and step.  An item without pairs (a
+    // tile no window reaches, when empty tiles are written) stores the zero sums.
     const int TC = a.TC;
-    const int CS = 2 * (TC + 1); // floats per coil
-    if (!empty) __syncthreads(); // the batch buffers are reused
-    if (!empty && has_block && (ri >> 1) < ncg) {
-#pragma unroll
-        for (int sb = 0; sb < 4; ++sb) {
-            // register v of lane j + 16 kg holds row 4 kg + v of slice sb: in 3-D x-y cell
-            // (kg, v) of z plane sb; in 2-D cell ((4 kg + v) >> 3, (4 kg + v) & 7) of x rows
-            // 2 sb, 2 sb + 1, i.e. x = 2 sb + (kg >> 1), y = 4 (kg & 1) + v
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int L0 = D == 3 ? bq0 * 4 + kg : bq0 * 8 + 2 * sb + (kg >> 1);
-                const int L1 = D == 3 ? bq1 * 4 + v : bq1 * 8 + 4 * (kg & 1) + v;
-                const int L2 = D == 3 ? bq2 * 4 + sb : 0;
-                s_out[(ri >> 1) * CS + ((L0 * T1 + L1) * T2 + L2) * 2 + (ri & 1)] = S[sb][v] - K[sb][v];
-            }
-        }
-    }
-    if (!empty) __syncthreads();
+    const int c = ri >> 1;         // coil within the group
+    const bool odd = (ri & 1) != 0;
+    const bool writer = c < ncg;
     int org[3] = {0, 0, 0}, teff[3] = {1, 1, 1};
     if (item.w < 0) {
         int r = item.x;
@@ -1131,51 +1114,37 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
             teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
     }
-    const int Tl = D == 3 ? T2 : T1;        // cells per row along the last axis
-    const int nrow = TC / Tl;                // rows per coil
-    const int hl = Tl >> 1;                  // cell pairs per row (Tl is even)
-    const float rhl = 1.0f / (float)hl, rnr = 1.0f / (float)nrow, rT1 = 1.0f / (float)T1;
-    const bool wide = item.w >= 0 || ((a.G | a.nf[D - 1]) & 1) == 0;
-    const int npairs = ncg * nrow * hl;
-    const bool pow2 = ((hl & (hl - 1)) | (nrow & (nrow - 1))) == 0;
-    const int lgh = __builtin_ctz(hl), lgr = __builtin_ctz(nrow);
-    for (int e = tid; e < npairs; e += blockDim.x) {
-        const int rowg = pow2 ? (e >> lgh) : fdiv_small(e, hl, rhl); // (coil, row)
-        const int h = e - rowg * hl;
-        const int coil = pow2 ? (rowg >> lgr) : fdiv_small(rowg, nrow, rnr);
-        const int row = rowg - coil * nrow;
-        const int cidx = row * Tl + 2 * h;
-        float2 c0 = make_float2(0.f, 0.f), c1 = c0;
-        if (!empty) {
-            const float *src = s_out + coil * CS + cidx * 2;
-            c0 = make_float2(src[0], src[1]);
-            c1 = make_float2(src[2], src[3]);
-        }
+    const bool wide = ((a.G | a.nf[D - 1]) & 1) == 0;
+    const int tl = teff[D - 1];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float o[4];
+#pragma unroll
+        for (int z = 0; z < 4; ++z) o[z] = D == 3 ? S[z][i] - K[z][i] : S[i][z] - K[i][z];
+        // quad_perm [1, 0, 3, 2]: the value of lane ^ 1
+        const float r0 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(odd ? o[0] : o[2]), 0xb1, 0xf, 0xf, false));
+        const float r1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(odd ? o[1] : o[3]), 0xb1, 0xf, 0xf, false));
+        const float4 w = odd ? make_float4(r0, o[2], r1, o[3]) : make_float4(o[0], r0, o[1], r1);
+        if (!writer) continue;
+        const int L0 = D == 3 ? bq0 * 4 + kg : bq0 * 8 + 2 * i + (kg >> 1);
+        const int L1 = D == 3 ? bq1 * 4 + i : bq1 * 8 + 4 * (kg & 1) + (odd ? 2 : 0);
+        const int L2 = D == 3 ? bq2 * 4 + (odd ? 2 : 0) : 0;
+        const int cidx = D == 3 ? (L0 * T1 + L1) * T2 + L2 : L0 * T1 + L1;
         if (item.w >= 0) {
-            float2 *dst = a.partial + ((int64_t)item.w * a.nc + cg0 + coil) * TC + cidx;
-            *(float4 *)dst = make_float4(c0.x, c0.y, c1.x, c1.y);
+            *(float4 *)(a.partial + ((int64_t)item.w * a.nc + cg0 + c) * TC + cidx) = w;
             continue;
         }
-        int L0, L1, L2 = 0;
-        if (D == 3) {
-            L0 = fdiv_small(row, T1, rT1);
-            L1 = row - L0 * T1;
-            L2 = 2 * h;
-        } else {
-            L0 = row;
-            L1 = 2 * h;
-        }
         if (L0 >= teff[0] || (D == 3 && L1 >= teff[1])) continue;
-        const int Ll = D == 3 ? L2 : L1, tl = teff[D - 1];
+        const int Ll = D == 3 ? L2 : L1;
         if (Ll >= tl) continue;
         int64_t gi = (int64_t)(org[0] + L0) * a.nf[1] + (org[1] + L1);
         if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
-        float2 *dst = a.grid + (int64_t)(cg0 + coil) * a.G + gi;
+        float2 *dst = a.grid + (int64_t)(cg0 + c) * a.G + gi;
         if (wide && Ll + 1 < tl) {
-            *(float4 *)dst = make_float4(c0.x, c0.y, c1.x, c1.y);
+            *(float4 *)dst = w;
         } else {
-            dst[0] = c0;
-            if (Ll + 1 < tl) dst[1] = c1;
+            dst[0] = make_float2(w.x, w.y);
+            if (Ll + 1 < tl) dst[1] = make_float2(w.z, w.w);
         }
     }
 }

@@ -34,6 +34,23 @@
 namespace gn {
 namespace {
 
+// Streaming 16-byte accesses: every grid element a pass reads or writes is touched
+// once per pass and the next pass runs over the whole grid, so the column passes load
+// and store, and pass A' loads, with the nontemporal hint (measured at 128^3 x 8
+// coils: fwd+adj 2.54 -> 2.40 ms).  Pass A's 8-byte grid stores and the spread's
+// tile stores were slower with it.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float2 *p)
+{
+    const f32x4_t t = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t *>(p));
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ void store_stream(float2 *p, float4 v)
+{
+    const f32x4_t t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<f32x4_t *>(p));
+}
+
 #ifndef GN_FFT_THREADS
 #define GN_FFT_THREADS 256
 #endif
@@ -278,7 +295,7 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
                 if (on0 || on1) {
                     const float2 *src = base + (int64_t)n * a.ncols + j;
                     if (v2) {
-                        v[u] = *reinterpret_cast<const float4 *>(src);
+                        v[u] = load_stream(src);
                         if (!on0) v[u].x = v[u].y = 0.f;
                         if (!on1) v[u].z = v[u].w = 0.f;
                     } else {
@@ -319,7 +336,7 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
         float2 *dst = base + (int64_t)n * a.ncols + j;
         if (v2) {
             const float2 lo = src[0], hi = src[1];
-            *reinterpret_cast<float4 *>(dst) = make_float4(lo.x, lo.y, hi.x, hi.y);
+            store_stream(dst, make_float4(lo.x, lo.y, hi.x, hi.y));
         } else {
             *dst = src[0];
         }
@@ -467,7 +484,7 @@ __device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t lin
             if (q < ngp) {
                 const float2 *src = a.grid + (c0 + c) * a.G + off + k;
                 if (gw) {
-                    v[u] = *reinterpret_cast<const float4 *>(src);
+                    v[u] = load_stream(src);
                 } else {
                     const float2 t = *src;
                     v[u] = make_float4(t.x, t.y, 0.f, 0.f);

@@ -36,9 +36,10 @@ namespace {
 
 // Streaming 16-byte accesses: every grid element a pass reads or writes is touched
 // once per pass and the next pass runs over the whole grid, so the column passes load
-// and store, and pass A' loads, with the nontemporal hint (measured at 128^3 x 8
-// coils: fwd+adj 2.54 -> 2.40 ms).  Pass A's 8-byte grid stores and the spread's
-// tile stores were slower with it.
+// and store, and pass A' loads its grid lines and stores its image pairs, with the
+// nontemporal hint (measured at 128^3 x 8 coils: fwd+adj 2.54 -> 2.39 ms).  Pass A's
+// 8-byte grid stores, the spread's tile stores and the gather's sample stores were
+// slower with it.
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 load_stream(const float2 *p)
 {
@@ -546,7 +547,7 @@ __device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t lin
         const float2 lo = get(n, c);
         if (iw) {
             const float2 hi = get(n + (cl ? 0 : 1), c + (cl ? 1 : 0));
-            *reinterpret_cast<float4 *>(dst) = make_float4(lo.x, lo.y, hi.x, hi.y);
+            store_stream(dst, make_float4(lo.x, lo.y, hi.x, hi.y));
         } else {
             *dst = lo;
         }

@@ -862,11 +862,12 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     constexpr int B = kSpreadBatch;
     extern __shared__ float lds_sp[];
     if (tile_skippable(a, item)) return; // uniform over the workgroup
-    // LDS: weight tables [B + 1][ttp] | reals [B + 1][ttp] (16 used) | masks [B][4] |
-    //      per-axis window parameters [B][4] {l, c, ns - j, t} | per-wave pair lists
+    // LDS: weight tables [B + 1][ttp] | reals [B + 1][ttp] (16 used) | masks [4][B] |
+    //      per-axis window parameters [4][B] {l, c, ns - j, t} | per-wave pair lists
     //      [waves][B]; row B of the tables and reals is the all-zero pair.  Table and
     //      value rows have the same stride, so a list entry is the byte offset of a
-    //      pair's rows in both.
+    //      pair's rows in both.  Masks and parameters are axis-major, so that a wave's
+    //      lanes touch consecutive entries.
     const int TT = a.TT;
     // table rows padded to a power of two: a pair's row starts at kk << lgt
     const int ttp = TT <= 16 ? 16 : TT <= 32 ? 32 : TT <= 64 ? 64 : 128;
@@ -1005,8 +1006,8 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
                 const int j = (pa.y >> (8 * ax)) & 0xff;
                 uint64_t m = ((1ull << c) - 1) << l;
                 m = (m | (m >> Ts[ax])) & ((1ull << Ts[ax]) - 1);
-                s_msk[pt * 4 + ax] = (int)(uint32_t)m;
-                s_par[pt * 4 + ax] = make_int4(l, c, a.ns - j, __float_as_int(ts[ax]));
+                s_msk[ax * B + pt] = (int)(uint32_t)m;
+                s_par[ax * B + pt] = make_int4(l, c, a.ns - j, __float_as_int(ts[ax]));
             }
             }
         }
@@ -1015,7 +1016,7 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         // keeps one column quad (its axis and 4 cells) and steps over the pairs
         if (e_i < TT) {
             for (int k = e_k0; k < n; k += e_kstep) {
-                const int4 par = s_par[k * 4 + e_ax];
+                const int4 par = s_par[e_ax * B + k];
                 int rel0 = e_L - par.x; // (L - l) mod T of the quad's first column
                 if (rel0 < 0) rel0 += e_T;
                 float w[4];
@@ -1034,9 +1035,9 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         if (!has_block) continue;
         uint32_t m0 = 0, m1 = 0, m2 = 0;
         if (lane < n) {
-            m0 = (uint32_t)s_msk[lane * 4];
-            m1 = (uint32_t)s_msk[lane * 4 + 1];
-            m2 = D == 3 ? (uint32_t)s_msk[lane * 4 + 2] : 1u;
+            m0 = (uint32_t)s_msk[lane];
+            m1 = (uint32_t)s_msk[B + lane];
+            m2 = D == 3 ? (uint32_t)s_msk[2 * B + lane] : 1u;
         }
         f32x4 acc[4];
 #pragma unroll

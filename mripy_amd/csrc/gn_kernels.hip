@@ -944,7 +944,7 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     const float e_coef = a.coef[e_ax];
     for (int base = item.y; base < item.z; base += B) {
         const int n = min(B, item.z - base);
-        __syncthreads(); // previous batch fully consumed
+        if (base != item.y) __syncthreads(); // previous batch fully consumed (uniform)
         // phase 1: per pair, a thread of the first wave loads the coil values and a
         // thread of the second wave (the same thread with one wave) builds the masks and
         // window parameters; each loads the pair's record

@@ -146,6 +146,16 @@ int gn_execute_type21_noalloc(gn_plan *plan, const void *in, void *out);
  * y = A(x) f; with f = dL/d(image) and w = y it is dL/dx for image = A(x)^H y. */
 int gn_traj_grad(gn_plan *plan, const void *image, const void *weights, float *grad);
 
+/* gn_traj_grad with flags (no counterpart in the reference; for autograd, whose
+ * backward of y = A(x) f follows the forward's type 2 of the same f).
+ * GN_TRAJ_REUSE_GRID: when the plan's last operation was GN_TYPE2 of this same
+ * image pointer, or a trajectory gradient of it, the plan's grid still holds that
+ * image's type-2 grid and the FFT passes are skipped; the caller guarantees that the
+ * image memory was not changed since.  Otherwise, and without the flag, the grid is
+ * recomputed as in gn_traj_grad.  Results are identical either way. */
+#define GN_TRAJ_REUSE_GRID 1
+int gn_traj_grad_ex(gn_plan *plan, const void *image, const void *weights, float *grad, int flags);
+
 /* Query: nf[3] oversampled grid, nspread, tau, number of samples, coils, and
  * tile[6] = spread (output) tile shape followed by the gather (home) tile shape. */
 int gn_plan_info(const gn_plan *plan, int64_t *nf, int *nspread, double *tau, int64_t *M, int64_t *ncoils,

@@ -50,6 +50,7 @@ SIGNATURES = {
     "gn_plan_stage_times": ([_vp, ctypes.POINTER(ctypes.c_double), _i64p, ctypes.c_int], ctypes.c_int),
     "gn_plan_sample_order": ([_vp, _ip], ctypes.c_int),
     "gn_traj_grad": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "gn_traj_grad_ex": ([_vp, _vp, _vp, _vp, ctypes.c_int], ctypes.c_int),
     "gn_plan_set_fft_engine": ([_vp, ctypes.c_int], ctypes.c_int),
     "gn_plan_destroy": ([_vp], ctypes.c_int),
     # include/gaussnufft_solver.h

@@ -48,10 +48,12 @@ class _Forward(torch.autograd.Function):
         plan = ctx.plan
         _bind(plan, ctx.coords)
         gy = gy.contiguous()
-        g_img = plan.adjoint(gy) if ctx.needs_input_grad[0] else None
         g_crd = None
         if ctx.needs_input_grad[1]:
-            g_crd = plan.traj_grad(image.detach(), gy)
+            # first: when nothing ran on the plan since the forward, its grid still holds
+            # the type-2 grid of image (saved, so alive and unmodified) and is reused
+            g_crd = plan.traj_grad(image.detach(), gy, reuse_grid=True)
+        g_img = plan.adjoint(gy) if ctx.needs_input_grad[0] else None
         return g_img, g_crd, None
 
 
@@ -74,7 +76,8 @@ class _Adjoint(torch.autograd.Function):
         g_y = plan.type2(g_img) if ctx.needs_input_grad[0] else None
         g_crd = None
         if ctx.needs_input_grad[1]:
-            g_crd = plan.traj_grad(g_img, y.detach())
+            # the type 2 just above left g_img's grid in the plan
+            g_crd = plan.traj_grad(g_img, y.detach(), reuse_grid=True)
         return g_y, g_crd, None
 
 

@@ -336,6 +336,7 @@ int gn_plan_set_tiles(gn_plan *p, const int *tile)
 {
     g_err.clear();
     if (!p || !tile) return fail(GN_EINVAL, "bad arguments");
+    p->grid_t2_img = nullptr;
     int old[3], oldg[3];
     memcpy(old, p->tile_override, sizeof(old));
     memcpy(oldg, p->gtile_override, sizeof(oldg));
@@ -363,6 +364,7 @@ int gn_plan_set_sense(gn_plan *p, const void *maps)
 {
     g_err.clear();
     if (!p) return fail(GN_EINVAL, "plan is null");
+    p->grid_t2_img = nullptr;
     if (!maps) {
         hipFree(p->sens);
         hipFree(p->sbuf);
@@ -407,6 +409,7 @@ static int execute_impl(gn_plan *p, int kind, const void *in, void *out, float2 
 {
     g_err.clear();
     if (!p) return fail(GN_EINVAL, "plan is null");
+    p->grid_t2_img = nullptr; // set again below after a plain type 2
     if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called before gn_execute");
     const bool sense = kind == GN_SENSE_TYPE1 || kind == GN_SENSE_TYPE2 || kind == GN_SENSE_ADJOINT ||
                        kind == GN_SENSE_TYPE21;
@@ -498,6 +501,7 @@ static int execute_impl(gn_plan *p, int kind, const void *in, void *out, float2 
     switch (kind) {
     case GN_TYPE2:
         rc = type2(in, out);
+        if (!rc && !sense) p->grid_t2_img = in;
         break;
     case GN_TYPE1:
         rc = type1(in, out, true, 1.0 / (double)p->M);
@@ -577,26 +581,37 @@ int gn_execute_type21_noalloc(gn_plan *p, const void *in, void *out)
 
 int gn_traj_grad(gn_plan *p, const void *image, const void *weights, float *grad)
 {
+    return gn_traj_grad_ex(p, image, weights, grad, 0);
+}
+
+int gn_traj_grad_ex(gn_plan *p, const void *image, const void *weights, float *grad, int flags)
+{
     g_err.clear();
     if (!p) return fail(GN_EINVAL, "plan is null");
     if (!p->has_points) return fail(GN_ESTATE, "gn_set_points must be called before gn_traj_grad");
     if (p->M == 0) return GN_OK;
     if (!image || !weights || !grad) return fail(GN_EINVAL, "null data pointer");
+    if (flags & ~GN_TRAJ_REUSE_GRID) return fail(GN_EINVAL, "unknown gn_traj_grad_ex flag");
     const Geometry &g = p->geo;
     const double pi = 3.141592653589793;
     const double c = std::pow(pi / p->tau, g.dim / 2.0);
     const float scale = (float)(c / (double)g.G);
     int rc;
-    if (p->fft_custom && p->fft_engine == GN_FFT_AUTO) {
+    if ((flags & GN_TRAJ_REUSE_GRID) && p->grid_t2_img == image) {
+        rc = GN_OK; // the grid already holds this image's type-2 grid
+    } else if (p->fft_custom && p->fft_engine == GN_FFT_AUTO) {
+        p->grid_t2_img = nullptr;
         const float sign2 = p->iflag >= 0 ? -1.0f : 1.0f;
         rc = timed(p, GN_STAGE_PAD, [&] { return pruned_type2_rows(p, (const float2 *)image, scale, sign2); });
         if (!rc) rc = timed(p, GN_STAGE_FFT2, [&] { return pruned_type2_cols(p, sign2); });
     } else {
+        p->grid_t2_img = nullptr;
         rc = p->fft_fwd ? GN_OK : create_fft(p);
         if (!rc) rc = timed(p, GN_STAGE_PAD, [&] { return launch_pad(p, (const float2 *)image, scale); });
         if (!rc) rc = timed(p, GN_STAGE_FFT2, [&] { return run_fft(p, p->iflag >= 0); });
     }
     if (!rc) rc = launch_traj_grad(p, (const float2 *)weights, grad);
+    p->grid_t2_img = rc ? nullptr : image;
     return rc;
 }
 
@@ -677,6 +692,7 @@ int gn_plan_set_fft_engine(gn_plan *p, int engine)
 {
     g_err.clear();
     if (!p) return fail(GN_EINVAL, "plan is null");
+    p->grid_t2_img = nullptr;
     if (engine != GN_FFT_AUTO && engine != GN_FFT_ROCFFT) return fail(GN_EINVAL, "unknown FFT engine");
     p->fft_engine = engine;
     return GN_OK;

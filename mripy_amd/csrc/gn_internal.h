@@ -240,6 +240,10 @@ struct gn_plan_s {
     gn::Records rec;
 
     float2 *grid = nullptr;   // nc * G oversampled grids, coil-major
+    // the image whose (non-SENSE) type-2 grid `grid` holds after the plan's last
+    // operation, or null: set by a GN_TYPE2 execution or a trajectory gradient, cleared
+    // by every other operation that may write the grid (gn_traj_grad_ex may reuse it)
+    const void *grid_t2_img = nullptr;
     float2 *ybuf = nullptr;   // nc * M intermediate samples (type 21)
     float2 *sens = nullptr;   // N * nc SENSE maps in the plan's layout (gn_plan_set_sense)
     float2 *sbuf = nullptr;   // N * nc coil images of a SENSE combine that does not fit one block

@@ -249,8 +249,12 @@ class Plan:
                                                src.data_ptr(), dst.data_ptr(), int(reps), t, ctypes.byref(ms)))
         return (tuple(t[:self.dim]), tuple(t[3:3 + self.dim])), ms.value
 
-    def traj_grad(self, image, weights):
-        """(dim, M) float32: sum_c Re(conj(w) d(type2 image)/dx) per sample and axis (analytic)."""
+    def traj_grad(self, image, weights, reuse_grid=False):
+        """(dim, M) float32: sum_c Re(conj(w) d(type2 image)/dx) per sample and axis (analytic).
+
+        reuse_grid: skip the FFT passes when the plan's last operation was type2 of this
+        same (live, unmodified) image tensor, whose type-2 grid the plan still holds; the
+        result is the same either way."""
         if self.M is None:
             raise RuntimeError("set_points must be called first")
         img = torch.as_tensor(image).to(device=self.device, dtype=torch.complex64).reshape(self.image_shape())
@@ -260,7 +264,8 @@ class Plan:
         out = torch.zeros((self.dim, self.M), dtype=torch.float32, device=self.device)
         with torch.cuda.device(self.device):
             self._bind_stream()
-            check(_lib.load().gn_traj_grad(self._h, img.data_ptr(), w.data_ptr(), out.data_ptr()))
+            check(_lib.load().gn_traj_grad_ex(self._h, img.data_ptr(), w.data_ptr(), out.data_ptr(),
+                                              1 if reuse_grid else 0))
         return out
 
     def sample_order(self):

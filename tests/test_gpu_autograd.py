@@ -51,6 +51,38 @@ def test_traj_grad_kernel_matches_fp64_analytic():
     assert rel(g, ref) <= 1e-4
 
 
+def test_traj_grad_grid_reuse_equals_recompute():
+    """gn_traj_grad_ex with GN_TRAJ_REUSE_GRID: after a type 2 of the same image it skips
+    the FFT passes; after any other operation, or for another image, it recomputes.  The
+    gradient is the same bit for bit in every case, and matches the FP64 analytic one."""
+    import torch
+    from mripy_amd.plan import Plan
+    print("case: traj_grad grid reuse", flush=True)
+    rng = np.random.default_rng(5)
+    M, nc, modes = 5000, 2, (16, 20, 18)
+    cs = [(np.pi * (2 * rng.random(M) - 1)).astype(np.float32) for _ in range(3)]
+    F = torch.tensor((rng.standard_normal(modes + (nc,)) + 1j * rng.standard_normal(modes + (nc,))).astype(np.complex64),
+                     device="cuda")
+    F2 = torch.tensor((rng.standard_normal(modes + (nc,)) + 1j * rng.standard_normal(modes + (nc,))).astype(np.complex64),
+                      device="cuda")
+    w = torch.tensor((rng.standard_normal((M, nc)) + 1j * rng.standard_normal((M, nc))).astype(np.complex64), device="cuda")
+    P = Plan(modes, ncoils=nc, rat=2, nspread=3)
+    P.set_points(cs)
+    g_full = P.traj_grad(F, w).cpu().numpy()
+    P.type2(F)
+    g_reuse = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()          # grid of F reused
+    g_again = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()          # left by the gradient itself
+    P.type2(F2)
+    g_other = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()          # grid of F2: recomputed
+    P.type2(F)
+    P.adjoint(w)                                                        # overwrites the grid
+    g_after_adj = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()
+    for g in (g_reuse, g_again, g_other, g_after_adj):
+        assert np.array_equal(g, g_full)
+    ref = O.traj_grad(cs, F.cpu().numpy(), w.cpu().numpy(), O.Params(modes, rat=2, nspread=3))
+    assert rel(g_full, ref) <= 1e-4
+
+
 def test_autograd_forward_map_gradients():
     import torch
     from mripy_amd.autograd import NUFFT

@@ -924,11 +924,9 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     if (tid < ttp) s_tab[B * ttp + tid] = 0.f;
     if (tid < 16) s_val[B * ttp + tid] = 0.f;
 
+    // Kahan sums and compensations: set by the first batch (S = batch sums, K = 0), or to
+    // zero after the loop when the item has no pairs
     float S[4][4], K[4][4];
-#pragma unroll
-    for (int sb = 0; sb < 4; ++sb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) S[sb][v] = K[sb][v] = 0.f;
 
     // phase-2 entry of this thread: the 4 consecutive table columns e_i .. e_i + 3 (one
     // axis e_ax, cells e_L .. e_L + 3: the tile extents are multiples of 4) of pairs
@@ -1080,7 +1078,10 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
 #pragma unroll
             for (int sb = 0; sb < 4; ++sb)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) S[sb][v] = acc[sb][v];
+                for (int v = 0; v < 4; ++v) {
+                    S[sb][v] = acc[sb][v];
+                    K[sb][v] = 0.f;
+                }
         } else {
 #pragma unroll
             for (int sb = 0; sb < 4; ++sb)
@@ -1115,8 +1116,35 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
             teff[ax] = (int)min((int64_t)a.tile[ax], a.nf[ax] - org[ax]);
         }
     }
+    if (item.y >= item.z || !has_block) { // no pairs: the tile's sums are zero
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[sb][v] = K[sb][v] = 0.f;
+    }
     const bool wide = ((a.G | a.nf[D - 1]) & 1) == 0;
     const int tl = teff[D - 1];
+    // this lane's cells of step i: 3-D (L0b, L1b + i, L2), 2-D (L0b + 2 i, L1b); the
+    // addresses of step 0 plus i times the step's stride
+    const int L0b = D == 3 ? bq0 * 4 + kg : bq0 * 8 + (kg >> 1);
+    const int L1b = D == 3 ? bq1 * 4 : bq1 * 8 + 4 * (kg & 1) + (odd ? 2 : 0);
+    const int L2 = D == 3 ? bq2 * 4 + (odd ? 2 : 0) : 0;
+    const int Ll = D == 3 ? L2 : L1b; // the pair's cell along the last axis
+    const bool slot = item.w >= 0;
+    float2 *dst0;
+    int64_t step;
+    bool lane_ok = writer;
+    if (slot) {
+        dst0 = a.partial + ((int64_t)item.w * a.nc + cg0 + c) * TC + (D == 3 ? (L0b * T1 + L1b) * T2 + L2 : L0b * T1 + L1b);
+        step = D == 3 ? T2 : 2 * T1;
+    } else {
+        lane_ok = lane_ok && Ll < tl && (D == 2 || L0b < teff[0]);
+        int64_t gi = (int64_t)(org[0] + L0b) * a.nf[1] + (org[1] + L1b);
+        if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
+        dst0 = a.grid + (int64_t)(cg0 + c) * a.G + gi;
+        step = D == 3 ? a.nf[2] : 2 * a.nf[1];
+    }
+    const bool pairw = slot || (wide && Ll + 1 < tl);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         float o[4];
@@ -1126,22 +1154,10 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
         const float r0 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(odd ? o[0] : o[2]), 0xb1, 0xf, 0xf, false));
         const float r1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(odd ? o[1] : o[3]), 0xb1, 0xf, 0xf, false));
         const float4 w = odd ? make_float4(r0, o[2], r1, o[3]) : make_float4(o[0], r0, o[1], r1);
-        if (!writer) continue;
-        const int L0 = D == 3 ? bq0 * 4 + kg : bq0 * 8 + 2 * i + (kg >> 1);
-        const int L1 = D == 3 ? bq1 * 4 + i : bq1 * 8 + 4 * (kg & 1) + (odd ? 2 : 0);
-        const int L2 = D == 3 ? bq2 * 4 + (odd ? 2 : 0) : 0;
-        const int cidx = D == 3 ? (L0 * T1 + L1) * T2 + L2 : L0 * T1 + L1;
-        if (item.w >= 0) {
-            *(float4 *)(a.partial + ((int64_t)item.w * a.nc + cg0 + c) * TC + cidx) = w;
-            continue;
-        }
-        if (L0 >= teff[0] || (D == 3 && L1 >= teff[1])) continue;
-        const int Ll = D == 3 ? L2 : L1;
-        if (Ll >= tl) continue;
-        int64_t gi = (int64_t)(org[0] + L0) * a.nf[1] + (org[1] + L1);
-        if (D == 3) gi = gi * a.nf[2] + (org[2] + L2);
-        float2 *dst = a.grid + (int64_t)(cg0 + c) * a.G + gi;
-        if (wide && Ll + 1 < tl) {
+        if (!lane_ok) continue;
+        if (!slot && (D == 3 ? L1b + i >= teff[1] : L0b + 2 * i >= teff[0])) continue;
+        float2 *dst = dst0 + i * step;
+        if (pairw) {
             *(float4 *)dst = w;
         } else {
             dst[0] = make_float2(w.x, w.y);

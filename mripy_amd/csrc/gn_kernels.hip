@@ -879,7 +879,7 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     int *s_list = (int *)(s_par + B * 4);
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6); // uniform: block set-up on the scalar unit
     const int lane = tid & 63;
     const int nwaves = blockDim.x >> 6;
     const bool split = nwaves >= 2; // phase 1 spread over two waves
@@ -892,15 +892,15 @@ __device__ __forceinline__ void spread_mfma_body(const SpreadArgs &a, const int4
     const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
     const int bidx = bz * nwaves + wave;
     const bool has_block = bidx < nb;
-    // block coordinates (bidx < 2^16: exact float-reciprocal division)
+    // block coordinates (wave-uniform integer arithmetic)
     int bq0, bq1, bq2 = 0;
     if (D == 3) {
-        const int q2 = fdiv_small(bidx, a.nblk[2], 1.0f / (float)a.nblk[2]);
+        const int q2 = bidx / a.nblk[2];
         bq2 = bidx - q2 * a.nblk[2];
-        bq0 = fdiv_small(q2, a.nblk[1], 1.0f / (float)a.nblk[1]);
+        bq0 = q2 / a.nblk[1];
         bq1 = q2 - bq0 * a.nblk[1];
     } else {
-        bq0 = fdiv_small(bidx, a.nblk[1], 1.0f / (float)a.nblk[1]);
+        bq0 = bidx / a.nblk[1];
         bq1 = bidx - bq0 * a.nblk[1];
     }
     // this lane's MFMA row ri (a cell of a slice) and its pair slot k = lane / 16

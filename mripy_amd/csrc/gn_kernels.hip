@@ -1208,9 +1208,10 @@ __device__ __forceinline__ void spread_reduce_body(const SpreadArgs &a, const in
     }
     const int TC = a.TC;
     const int T1 = a.tile[1], T2 = (D == 3) ? a.tile[2] : 1;
-    for (int64_t e = threadIdx.x; e < (int64_t)TC * a.nc; e += blockDim.x) {
+    const int ne = (int)(TC * a.nc); // a tile's cells of every coil (< 2^31)
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
         const int64_t coil = e / TC;
-        const int cidx = (int)(e % TC);
+        const int cidx = e - (int)coil * TC;
         int x0, x1, x2 = 0;
         if (D == 3) {
             x2 = cidx % T2;

@@ -70,10 +70,20 @@ def test_traj_grad_grid_reuse_equals_recompute():
     P.set_points(cs)
     g_full = P.traj_grad(F, w).cpu().numpy()
     P.type2(F)
+    P.set_timing(True)
+    P.stage_times(reset=True)
     g_reuse = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()          # grid of F reused
+    st = P.stage_times(reset=True)
+    P.set_timing(False)
+    assert st["pad"][1] == 0 and st["fft2"][1] == 0                     # no FFT pass ran
     g_again = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()          # left by the gradient itself
     P.type2(F2)
+    P.set_timing(True)
+    P.stage_times(reset=True)
     g_other = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()          # grid of F2: recomputed
+    st = P.stage_times(reset=True)
+    P.set_timing(False)
+    assert st["pad"][1] >= 1 and st["fft2"][1] >= 1
     P.type2(F)
     P.adjoint(w)                                                        # overwrites the grid
     g_after_adj = P.traj_grad(F, w, reuse_grid=True).cpu().numpy()

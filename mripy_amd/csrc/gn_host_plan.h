@@ -109,7 +109,8 @@ inline void fft_tables(int L, const std::vector<int> &r, std::vector<int> &pinv,
 }
 
 struct WorkItems {
-    std::vector<Item> spread;  // {tile, pbeg, pend, slot or -1}
+    std::vector<Item> spread;  // {tile, pbeg, pend, slot or -1}: tiles with pairs, then empty tiles
+    int full = 0;              // spread items of tiles with pairs (the first part)
     std::vector<Item> reduce;  // {tile, slot0, nslots, 0}
     std::vector<Item> gather;  // {gather tile, sbeg, send, 0}
     std::vector<unsigned long long> occ; // per column of spread tiles: leading-axis tile groups with pairs
@@ -131,16 +132,20 @@ inline void build_gather_items(int64_t Kg, const int *h_home, int chunk, std::ve
 // Work items from the bin offsets: h_pair[K + 1] (pairs of each spread tile) and
 // h_home[Kg + 1] (samples of each gather tile).  Every spread tile gets at least one
 // item; tiles with more than spread_chunk pairs are split into slices written to
-// partial slots and combined by a reduce item.  ntiles: spread tiles per axis (1 for
-// absent axes).
+// partial slots and combined by a reduce item.  The items of tiles with pairs come
+// first (w.full of them), in tile order, then one item per empty tile, in tile order.
+// ntiles: spread tiles per axis (1 for absent axes).
 inline void build_work_items(int64_t K, int64_t Kg, const int *ntiles, const int *h_pair, const int *h_home,
                              int spread_chunk, int gather_chunk, WorkItems &w)
 {
     w = WorkItems();
+    std::vector<Item> empty;
     for (int64_t k = 0; k < K; ++k) {
         const int b = h_pair[k], e = h_pair[k + 1];
         const int n = e - b;
-        if (n <= spread_chunk) {
+        if (n == 0) {
+            empty.push_back({(int)k, b, e, -1});
+        } else if (n <= spread_chunk) {
             w.spread.push_back({(int)k, b, e, -1});
         } else {
             const int nch = (n + spread_chunk - 1) / spread_chunk;
@@ -150,6 +155,8 @@ inline void build_work_items(int64_t K, int64_t Kg, const int *ntiles, const int
             w.slots += nch;
         }
     }
+    w.full = (int)w.spread.size();
+    w.spread.insert(w.spread.end(), empty.begin(), empty.end());
     build_gather_items(Kg, h_home, gather_chunk, w.gather);
     // one mask word per column of spread tiles; bit b: leading-axis tiles [b k, (b + 1) k)
     const int64_t ncolt = (int64_t)ntiles[1] * ntiles[2];

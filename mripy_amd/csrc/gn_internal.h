@@ -123,6 +123,7 @@ struct Records {
     // work items
     int4 *spread_items = nullptr; // {tile, pbeg, pend, slot or -1}
     int n_spread_items = 0;
+    int n_spread_full = 0; // the first n_spread_full items: tiles with pairs; then one per empty tile
     int4 *reduce_items = nullptr; // {tile, slot0, nslots, 0}
     int n_reduce_items = 0;
     int n_slots = 0;

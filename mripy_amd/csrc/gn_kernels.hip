@@ -505,31 +505,40 @@ __global__ __launch_bounds__(256) void k_pair_records(int64_t P, const int *__re
 // Spread work items, built on the device from the tile offsets pstart[K + 1] (the
 // rule of the host planner's build_work_items): tile k with n pairs gets one item
 // {k, pbeg, pend, -1}, or, when n > chunk, ceil(n / chunk) slices {k, ..., slot} that
-// write consecutive partial slots and one reduce item {k, slot0, slices, 0}.  Items,
-// slots and reduce items are numbered in tile order by exclusive scans of the
-// per-tile counts that k_item_counts writes.
+// write consecutive partial slots and one reduce item {k, slot0, slices, 0}.  The
+// items of tiles with pairs come first, in tile order, then one item per empty tile,
+// in tile order (a launch that need not write empty tiles takes only the first
+// part).  Items, empty-tile items, slots and reduce items are numbered by exclusive
+// scans of the per-tile counts that k_item_counts writes.
 __global__ __launch_bounds__(256) void k_item_counts(int64_t K, const int *__restrict__ pstart, int chunk,
-                                                     int *__restrict__ nitems, int *__restrict__ nslots,
-                                                     int *__restrict__ nsplit)
+                                                     int *__restrict__ nitems, int *__restrict__ nempty,
+                                                     int *__restrict__ nslots, int *__restrict__ nsplit)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const int n = pstart[k + 1] - pstart[k];
     const int nch = (n + chunk - 1) / chunk;
     const bool split = n > chunk;
-    nitems[k] = split ? nch : 1;
+    nitems[k] = split ? nch : n > 0 ? 1 : 0;
+    nempty[k] = n == 0 ? 1 : 0;
     nslots[k] = split ? nch : 0;
     nsplit[k] = split ? 1 : 0;
 }
 
+// nfull: the items of tiles with pairs (grand total of the ioff scan)
 __global__ __launch_bounds__(256) void k_item_write(int64_t K, const int *__restrict__ pstart, int chunk,
-                                                    const int *__restrict__ ioff, const int *__restrict__ soff,
+                                                    const int *__restrict__ ioff, const int *__restrict__ eoff,
+                                                    const int *__restrict__ nfull, const int *__restrict__ soff,
                                                     const int *__restrict__ roff, int4 *__restrict__ items,
                                                     int4 *__restrict__ reduce)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const int b = pstart[k], e = pstart[k + 1];
+    if (e == b) {
+        items[*nfull + eoff[k]] = make_int4((int)k, b, e, -1);
+        return;
+    }
     const int o = ioff[k];
     if (e - b <= chunk) {
         items[o] = make_int4((int)k, b, e, -1);
@@ -2140,7 +2149,7 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
 
     Scratch sc(st);
     std::vector<int> h_home(Kg + 1, 0);
-    int counts[3] = {0, 0, 0}; // spread items, partial slots, reduce items
+    int counts[4] = {0, 0, 0, 0}; // items of tiles with pairs, partial slots, reduce items, empty tiles
     int *tile_pairs = nullptr; // K + 1 offsets of each spread tile's pairs (scratch)
 
     if (M > 0) {
@@ -2236,19 +2245,22 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
             (rc = grow_alloc(&R.occ_mask, &R.occ_mask_cap, (size_t)ncolt, "tile masks")))
             return rc;
         const int64_t nbk = (K + kScanTile - 1) / kScanTile;
-        int *ni = nullptr, *ns = nullptr, *nr = nullptr, *bi = nullptr, *bs = nullptr, *br = nullptr;
-        if ((rc = sc.alloc(&ni, (size_t)K, "work items")) || (rc = sc.alloc(&ns, (size_t)K, "work items")) ||
-            (rc = sc.alloc(&nr, (size_t)K, "work items")) || (rc = sc.alloc(&bi, (size_t)nbk + 1, "work items")) ||
+        int *ni = nullptr, *ne = nullptr, *ns = nullptr, *nr = nullptr;
+        int *bi = nullptr, *be = nullptr, *bs = nullptr, *br = nullptr;
+        if ((rc = sc.alloc(&ni, (size_t)K, "work items")) || (rc = sc.alloc(&ne, (size_t)K, "work items")) ||
+            (rc = sc.alloc(&ns, (size_t)K, "work items")) || (rc = sc.alloc(&nr, (size_t)K, "work items")) ||
+            (rc = sc.alloc(&bi, (size_t)nbk + 1, "work items")) || (rc = sc.alloc(&be, (size_t)nbk + 1, "work items")) ||
             (rc = sc.alloc(&bs, (size_t)nbk + 1, "work items")) || (rc = sc.alloc(&br, (size_t)nbk + 1, "work items")))
             return rc;
         const dim3 gk(grid_blocks(K, 256, INT32_MAX));
-        hipLaunchKernelGGL(k_item_counts, gk, dim3(256), 0, st, K, tile_pairs, kSpreadChunk, ni, ns, nr);
+        hipLaunchKernelGGL(k_item_counts, gk, dim3(256), 0, st, K, tile_pairs, kSpreadChunk, ni, ne, ns, nr);
         GN_HIP(hipGetLastError());
         GN_HIP(device_scan(ni, K, bi, st));
+        GN_HIP(device_scan(ne, K, be, st));
         GN_HIP(device_scan(ns, K, bs, st));
         GN_HIP(device_scan(nr, K, br, st));
-        hipLaunchKernelGGL(k_item_write, gk, dim3(256), 0, st, K, tile_pairs, kSpreadChunk, ni, ns, nr, R.spread_items,
-                           R.reduce_items);
+        hipLaunchKernelGGL(k_item_write, gk, dim3(256), 0, st, K, tile_pairs, kSpreadChunk, ni, ne, bi + nbk, ns, nr,
+                           R.spread_items, R.reduce_items);
         GN_HIP(hipGetLastError());
         R.mask_k = (g.ntiles[0] + 63) / 64;
         hipLaunchKernelGGL(k_occ_masks, dim3(grid_blocks(ncolt, 256, INT32_MAX)), dim3(256), 0, st, ncolt, g.ntiles[0],
@@ -2257,6 +2269,7 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
         GN_HIP(hipMemcpyAsync(&counts[0], bi + nbk, sizeof(int), hipMemcpyDeviceToHost, st));
         GN_HIP(hipMemcpyAsync(&counts[1], bs + nbk, sizeof(int), hipMemcpyDeviceToHost, st));
         GN_HIP(hipMemcpyAsync(&counts[2], br + nbk, sizeof(int), hipMemcpyDeviceToHost, st));
+        GN_HIP(hipMemcpyAsync(&counts[3], be + nbk, sizeof(int), hipMemcpyDeviceToHost, st));
         GN_HIP(hipMemcpyAsync(h_home.data(), R.bin_start, sizeof(int) * (Kg + 1), hipMemcpyDeviceToHost, st));
         GN_HIP(hipStreamSynchronize(st));
     }
@@ -2269,7 +2282,8 @@ int bin_and_sort(gn_plan_s *p, int64_t M, const void *x, const void *y, const vo
     if ((rc = grow_alloc(&R.gather_items, &R.gather_items_cap, gitems.size(), "work items"))) return rc;
     if (!gitems.empty())
         GN_HIP(hipMemcpy(R.gather_items, gitems.data(), sizeof(int4) * gitems.size(), hipMemcpyHostToDevice));
-    R.n_spread_items = counts[0];
+    R.n_spread_items = counts[0] + counts[3];
+    R.n_spread_full = counts[0];
     R.n_slots = counts[1];
     R.n_reduce_items = counts[2];
     R.n_gather_items = (int)gitems.size();
@@ -2340,6 +2354,7 @@ hipError_t spread_launch(const SpreadArgs &a, int nitems, int64_t nc, hipStream_
     const int nb = a.nblk[0] * a.nblk[1] * (D == 3 ? a.nblk[2] : 1);
     const int waves = std::min(nb, kSpreadWavesMax);
     const int nbg = (nb + waves - 1) / waves;
+    if (nitems == 0) return hipSuccess;
     const dim3 grid((unsigned)nitems, (unsigned)ng, (unsigned)nbg);
     if (!grid_ok(grid)) return hipErrorInvalidConfiguration;
     if (mfma_tiles_ok(a, D)) {
@@ -2496,15 +2511,18 @@ int launch_spread(gn_plan_s *p, const float2 *samples, int use_dcf, float2 *grid
 {
     const Geometry &g = p->geo;
     SpreadArgs a = make_spread_args(p, samples, use_dcf);
+    int nitems = p->rec.n_spread_items;
     if (grid) {
         a.grid = grid; // every tile written (no pass skips cells of this grid)
     } else if (p->fft_custom && p->fft_engine == GN_FFT_AUTO) {
-        // the pruned pass C' reads only occupied tiles: empty ones need no zeros
+        // the pruned pass C' reads only occupied tiles: empty ones need no zeros.  With
+        // one mask bit per tile every empty tile is skipped, so the empty tiles' items
+        // (after the others) are not launched at all.
         a.occ = p->rec.occ_mask;
         a.occ_k = p->rec.mask_k;
+        if (p->rec.mask_k == 1) nitems = p->rec.n_spread_full;
     }
-    hipError_t e = spread_any(g.dim, g.ns, a, p->rec.n_spread_items, p->rec.n_reduce_items, p->rec.reduce_items, p->nc,
-                              p->stream);
+    hipError_t e = spread_any(g.dim, g.ns, a, nitems, p->rec.n_reduce_items, p->rec.reduce_items, p->nc, p->stream);
     if (e != hipSuccess) return hip_fail(e, "spread launch");
     return GN_OK;
 }

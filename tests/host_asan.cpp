@@ -104,6 +104,16 @@ static void check_items(int dim, const int64_t *modes, int ns, int64_t M, int ra
         }
     }
     for (int64_t k = 0; k < K; ++k) CHECK(covered[(size_t)k] == hp[(size_t)k + 1] - hp[(size_t)k]);
+    // tiles with pairs first, in tile order; then exactly one item per empty tile, in tile order
+    CHECK(w.full >= 0 && w.full <= (int)w.spread.size());
+    int64_t nempty = 0;
+    for (int64_t k = 0; k < K; ++k) nempty += hp[(size_t)k + 1] == hp[(size_t)k];
+    CHECK((int64_t)w.spread.size() - w.full == nempty);
+    for (size_t i = 0; i < w.spread.size(); ++i) {
+        const Item &it = w.spread[i];
+        CHECK(((int)i < w.full) == (hp[(size_t)it.x + 1] > hp[(size_t)it.x]));
+        if (i > 0 && (int)i != w.full) CHECK(w.spread[i - 1].x <= it.x);
+    }
     CHECK((int)slots.size() == w.slots);
     for (const Item &it : w.reduce) CHECK(it.y >= 0 && it.y + it.z <= w.slots && it.z >= 2);
     std::vector<int64_t> gcov((size_t)Kg, 0);

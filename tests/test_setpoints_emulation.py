@@ -308,11 +308,14 @@ def test_row_staging_covers_region_once(modes, ns, rat):
 
 
 def spread_items_host(pstart, chunk):
-    """build_work_items (gn_host_plan.h), spread part: items, reduce items, slots."""
-    items, red, slots = [], [], 0
+    """build_work_items (gn_host_plan.h), spread part: items (tiles with pairs, then the
+    empty tiles), reduce items, slots."""
+    items, empty, red, slots = [], [], [], 0
     for k in range(len(pstart) - 1):
         b, e = int(pstart[k]), int(pstart[k + 1])
-        if e - b <= chunk:
+        if e == b:
+            empty.append((k, b, e, -1))
+        elif e - b <= chunk:
             items.append((k, b, e, -1))
         else:
             nch = -(-(e - b) // chunk)
@@ -320,25 +323,31 @@ def spread_items_host(pstart, chunk):
             for c in range(nch):
                 items.append((k, b + c * chunk, min(e, b + (c + 1) * chunk), slots + c))
             slots += nch
-    return items, red, slots
+    return items + empty, red, slots
 
 
 def spread_items_device(pstart, chunk):
-    """k_item_counts, three exclusive scans and k_item_write (gn_kernels.hip)."""
+    """k_item_counts, four exclusive scans and k_item_write (gn_kernels.hip)."""
     K = len(pstart) - 1
     n = np.diff(pstart)
     nch = -(-n // chunk)
     split = n > chunk
-    ni = np.where(split, nch, 1)
+    ni = np.where(split, nch, np.where(n > 0, 1, 0))
+    ne = (n == 0).astype(np.int64)
     ns = np.where(split, nch, 0)
     nr = split.astype(np.int64)
     ioff = np.concatenate([[0], np.cumsum(ni)])
+    eoff = np.concatenate([[0], np.cumsum(ne)])
     soff = np.concatenate([[0], np.cumsum(ns)])
     roff = np.concatenate([[0], np.cumsum(nr)])
-    items = [None] * int(ioff[-1])
+    nfull = int(ioff[-1])
+    items = [None] * (nfull + int(eoff[-1]))
     red = [None] * int(roff[-1])
     for k in range(K):
         b, e = int(pstart[k]), int(pstart[k + 1])
+        if e == b:
+            items[nfull + int(eoff[k])] = (k, b, e, -1)
+            continue
         o = int(ioff[k])
         if e - b <= chunk:
             items[o] = (k, b, e, -1)

@@ -264,27 +264,14 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
         mk1 = (v2 && j + 1 < ncb) ? col_mask(a, a.rmask, col0 + j + 1) : mk0;
     }
     int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
-    load_table(tw, a.tw, a.L, a.sign);
-    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
-    __syncthreads();
-    // pruned input: the FFT positions no input row maps to, [rows_in - half_in,
-    // L - half_in) with in_map and [rows_in, L) without, start as zeros; the loads
-    // below write every other slot, so no barrier separates the two
-    if (a.rows_in < a.L) {
-        const int pz = a.in_map ? a.rows_in - half_in : a.rows_in;
-        const int nz = (a.L - a.rows_in) << cbs;
-        for (int e = threadIdx.x; e < nz; e += blockDim.x) {
-            const int r = e >> cbs, j = e & (CB - 1);
-            buf[spinv[pz + r] * estride + j] = make_float2(0.f, 0.f);
-        }
-    }
     // kLoadUnroll independent global loads in flight per thread before the LDS stores
-    for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
-        float4 v[kLoadUnroll];
+    float4 v[kLoadUnroll];
+    auto load_chunk = [&](int e0) {
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;
             const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
+            v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (e < total_in && j < ncb) {
                 bool on0 = true, on1 = true;
                 if (MASK == 1) { // cells of tiles the spread did not write enter as zeros
@@ -292,7 +279,6 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
                     on0 = (mk0 >> b) & 1ull;
                     on1 = (mk1 >> b) & 1ull;
                 }
-                v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (on0 || on1) {
                     const float2 *src = base + (int64_t)n * a.ncols + j;
                     if (v2) {
@@ -306,6 +292,28 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
                 }
             }
         }
+    };
+    // Passes with pruned input (the type-2 column passes) issue the first chunk's loads
+    // before the tables are staged, so that the grid and table latencies overlap.  The
+    // type-1 column passes, which read every row, were slower that way.
+    const bool early = MASK == 0 && a.rows_in < a.L;
+    if (early) load_chunk(threadIdx.x);
+    load_table(tw, a.tw, a.L, a.sign);
+    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
+    __syncthreads();
+    // pruned input: the FFT positions no input row maps to, [rows_in - half_in,
+    // L - half_in) with in_map and [rows_in, L) without, start as zeros; the stores
+    // below write every other slot, so no barrier separates the two
+    if (a.rows_in < a.L) {
+        const int pz = a.in_map ? a.rows_in - half_in : a.rows_in;
+        const int nz = (a.L - a.rows_in) << cbs;
+        for (int e = threadIdx.x; e < nz; e += blockDim.x) {
+            const int r = e >> cbs, j = e & (CB - 1);
+            buf[spinv[pz + r] * estride + j] = make_float2(0.f, 0.f);
+        }
+    }
+    for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
+        if (!early || e0 != (int)threadIdx.x) load_chunk(e0);
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;

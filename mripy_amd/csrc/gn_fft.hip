@@ -267,8 +267,12 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
     // kLoadUnroll independent global loads in flight per thread before the LDS stores
     float4 v[kLoadUnroll];
     auto load_chunk = [&](int e0) {
+        // elements left from this chunk's first (uniform): steps past the end are
+        // skipped on a scalar branch
+        const int left = __builtin_amdgcn_readfirstlane(total_in - (e0 - (int)threadIdx.x));
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
+            if (u * (int)blockDim.x >= left) break;
             const int e = e0 + u * blockDim.x;
             const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
             v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -314,8 +318,10 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
     }
     for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
         if (!early || e0 != (int)threadIdx.x) load_chunk(e0);
+        const int left = __builtin_amdgcn_readfirstlane(total_in - (e0 - (int)threadIdx.x));
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; ++u) {
+            if (u * (int)blockDim.x >= left) break;
             const int e = e0 + u * blockDim.x;
             const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
             if (e < total_in && j < ncb) {

@@ -63,6 +63,8 @@ __device__ __forceinline__ void store_stream(float2 *p, float4 v)
 #endif
 constexpr int kFFTThreads = GN_FFT_THREADS;       // row passes
 constexpr int kLoadUnroll = 8;                    // global loads in flight per thread
+constexpr int kColLoadUnroll = 4;                 // the same in the column passes (a 512-thread block
+                                                  // loads 256 x 8 cell pairs in one step)
 constexpr int kFFTColThreads = GN_FFT_COL_THREADS; // column passes
 static_assert((GN_FFT_CB & (GN_FFT_CB - 1)) == 0, "column blocks index rows with shifts");
 
@@ -264,34 +266,52 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
         mk1 = (v2 && j + 1 < ncb) ? col_mask(a, a.rmask, col0 + j + 1) : mk0;
     }
     int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
-    // kLoadUnroll independent global loads in flight per thread before the LDS stores
-    float4 v[kLoadUnroll];
+    // kColLoadUnroll independent global loads in flight per thread before the LDS stores.
+    // 16- and 8-byte cells take separate loops, and the mask is applied at the LDS
+    // stores: a loaded value that is copied or selected right after its load makes the
+    // wave wait for that load before it issues the next one.
+    float4 v[kColLoadUnroll];
+#pragma unroll
+    for (int u = 0; u < kColLoadUnroll; ++u) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t onb = 0; // masked pass: bit 2u (2u + 1) set when column j (j + 1) of step u lies in a written tile
     auto load_chunk = [&](int e0) {
         // elements left from this chunk's first (uniform): steps past the end are
         // skipped on a scalar branch
         const int left = __builtin_amdgcn_readfirstlane(total_in - (e0 - (int)threadIdx.x));
+        onb = 0;
+        if (v2) {
 #pragma unroll
-        for (int u = 0; u < kLoadUnroll; ++u) {
-            if (u * (int)blockDim.x >= left) break;
-            const int e = e0 + u * blockDim.x;
-            const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
-            v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (e < total_in && j < ncb) {
-                bool on0 = true, on1 = true;
-                if (MASK == 1) { // cells of tiles the spread did not write enter as zeros
-                    const int b = fdiv(n, a.mrow, rmrow);
-                    on0 = (mk0 >> b) & 1ull;
-                    on1 = (mk1 >> b) & 1ull;
+            for (int u = 0; u < kColLoadUnroll; ++u) {
+                if (u * (int)blockDim.x >= left) break;
+                const int e = e0 + u * blockDim.x;
+                const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
+                if (e < total_in && j < ncb) {
+                    uint32_t on = 3u;
+                    if (MASK == 1) { // cells of tiles the spread did not write enter as zeros
+                        const int b = fdiv(n, a.mrow, rmrow);
+                        on = (uint32_t)((mk0 >> b) & 1ull) | ((uint32_t)((mk1 >> b) & 1ull) << 1);
+                        onb |= on << (2 * u);
+                    }
+                    if (on) v[u] = load_stream(base + (int64_t)n * a.ncols + j);
                 }
-                if (on0 || on1) {
-                    const float2 *src = base + (int64_t)n * a.ncols + j;
-                    if (v2) {
-                        v[u] = load_stream(src);
-                        if (!on0) v[u].x = v[u].y = 0.f;
-                        if (!on1) v[u].z = v[u].w = 0.f;
-                    } else {
-                        const float2 t = *src;
-                        v[u] = make_float4(t.x, t.y, 0.f, 0.f);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kColLoadUnroll; ++u) {
+                if (u * (int)blockDim.x >= left) break;
+                const int e = e0 + u * blockDim.x;
+                const int n = e >> cbws, j = e & (cbw - 1);
+                if (e < total_in && j < ncb) {
+                    uint32_t on = 1u;
+                    if (MASK == 1) {
+                        const int b = fdiv(n, a.mrow, rmrow);
+                        on = (uint32_t)((mk0 >> b) & 1ull);
+                        onb |= on << (2 * u);
+                    }
+                    if (on) {
+                        const float2 t = base[(int64_t)n * a.ncols + j];
+                        v[u].x = t.x;
+                        v[u].y = t.y;
                     }
                 }
             }
@@ -316,11 +336,11 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
             buf[spinv[pz + r] * estride + j] = make_float2(0.f, 0.f);
         }
     }
-    for (int e0 = threadIdx.x; e0 < total_in; e0 += kLoadUnroll * blockDim.x) {
+    for (int e0 = threadIdx.x; e0 < total_in; e0 += kColLoadUnroll * blockDim.x) {
         if (!early || e0 != (int)threadIdx.x) load_chunk(e0);
         const int left = __builtin_amdgcn_readfirstlane(total_in - (e0 - (int)threadIdx.x));
 #pragma unroll
-        for (int u = 0; u < kLoadUnroll; ++u) {
+        for (int u = 0; u < kColLoadUnroll; ++u) {
             if (u * (int)blockDim.x >= left) break;
             const int e = e0 + u * blockDim.x;
             const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
@@ -331,8 +351,13 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
                     if (p < 0) p += a.L;
                 }
                 float2 *dst = buf + spinv[p] * estride + j;
-                dst[0] = make_float2(v[u].x, v[u].y);
-                if (v2) dst[1] = make_float2(v[u].z, v[u].w);
+                float4 w = v[u];
+                if (MASK == 1) {
+                    if (!((onb >> (2 * u)) & 1u)) w.x = w.y = 0.f;
+                    if (!((onb >> (2 * u + 1)) & 1u)) w.z = w.w = 0.f;
+                }
+                dst[0] = make_float2(w.x, w.y);
+                if (v2) dst[1] = make_float2(w.z, w.w);
             }
         }
     }

@@ -63,6 +63,8 @@ __device__ __forceinline__ void store_stream(float2 *p, float4 v)
 #endif
 constexpr int kFFTThreads = GN_FFT_THREADS;       // row passes
 constexpr int kLoadUnroll = 8;                    // global loads in flight per thread
+constexpr int kPadLoadUnroll = 4;                 // the same in pass A (a 256-thread block loads
+                                                  // 128 x 8 cells in one step)
 constexpr int kColLoadUnroll = 4;                 // the same in the column passes (a 512-thread block
                                                   // loads 256 x 8 cell pairs in one step)
 constexpr int kFFTColThreads = GN_FFT_COL_THREADS; // column passes
@@ -450,11 +452,20 @@ __device__ __forceinline__ void fft_rows_pad_body(const RowArgs &a, int64_t line
     const int half = a.m / 2;
     const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
     const int total = a.m * ncl;
-    for (int e0 = threadIdx.x; e0 < total; e0 += kLoadUnroll * blockDim.x) {
-        float2 v[kLoadUnroll];
-        int nn[kLoadUnroll], cc[kLoadUnroll];
+    // The image values and their deapodization factors of kPadLoadUnroll steps are loaded
+    // together; the SENSE product takes a loop of its own, so that no loaded value is
+    // used (which makes the wave wait for it) before the last load of the step issues.
+    float2 v[kPadLoadUnroll];
+    float dl[kPadLoadUnroll];
 #pragma unroll
-        for (int u = 0; u < kLoadUnroll; ++u) {
+    for (int u = 0; u < kPadLoadUnroll; ++u) {
+        v[u] = make_float2(0.f, 0.f);
+        dl[u] = 0.f;
+    }
+    for (int e0 = threadIdx.x; e0 < total; e0 += kPadLoadUnroll * blockDim.x) {
+        int nn[kPadLoadUnroll], cc[kPadLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kPadLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;
             int n, c;
             if (a.layout == GN_COIL_LAST) {
@@ -466,23 +477,36 @@ __device__ __forceinline__ void fft_rows_pad_body(const RowArgs &a, int64_t line
             }
             nn[u] = n;
             cc[u] = c;
-            if (e < total) {
-                if (a.sens) { // SENSE: coil c's line is x * S_c
+        }
+        if (a.sens) { // SENSE: coil c's line is x * S_c
+#pragma unroll
+            for (int u = 0; u < kPadLoadUnroll; ++u) {
+                const int e = e0 + u * blockDim.x, n = nn[u], c = cc[u];
+                if (e < total) {
                     const float2 x = a.img_in[line * a.m + n];
                     const float2 sc = a.layout == GN_COIL_LAST ? a.sens[(line * a.m + n) * a.nc + c0 + c]
                                                                : a.sens[(c0 + c) * a.nlines * a.m + line * a.m + n];
                     v[u] = cmul(x, sc);
-                } else {
-                    v[u] = a.layout == GN_COIL_LAST ? a.img_in[(line * a.m + n) * a.nc + c0 + c]
-                                                    : a.img_in[(c0 + c) * a.nlines * a.m + line * a.m + n];
+                    dl[u] = a.deap_last[n];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kPadLoadUnroll; ++u) {
+                const int e = e0 + u * blockDim.x, n = nn[u], c = cc[u];
+                if (e < total) {
+                    const int64_t src = a.layout == GN_COIL_LAST ? (line * a.m + n) * a.nc + c0 + c
+                                                                 : (c0 + c) * a.nlines * a.m + line * a.m + n;
+                    v[u] = a.img_in[src];
+                    dl[u] = a.deap_last[n];
                 }
             }
         }
 #pragma unroll
-        for (int u = 0; u < kLoadUnroll; ++u) {
+        for (int u = 0; u < kPadLoadUnroll; ++u) {
             const int e = e0 + u * blockDim.x;
             if (e < total) {
-                const float d = d01 * a.deap_last[nn[u]];
+                const float d = d01 * dl[u];
                 int p = nn[u] - half;
                 if (p < 0) p += a.L;
                 buf[cc[u] * ls + spinv[p]] = make_float2(v[u].x * d, v[u].y * d);

@@ -209,13 +209,16 @@ __device__ __forceinline__ void fft_lines(float2 *buf, int nlines, int lstride, 
     }
 }
 
-// Twiddles of a transform of this sign: the table holds e^{-2 pi i k/L}; sign -1
-// keeps it, +1 conjugates it.
-__device__ inline void load_table(float2 *dst, const float2 *src, int n, float sign)
+// Stage the twiddles of a transform of this sign (the table holds e^{-2 pi i k/L}; sign
+// -1 keeps it, +1 conjugates it) and the digit-reversal map of a length-n transform in
+// LDS.  Both loads of an entry are issued before either is used.
+__device__ inline void load_tables(float2 *tw_dst, int *pinv_dst, const float2 *tw, const int *pinv, int n, float sign)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const float2 w = src[i];
-        dst[i] = make_float2(w.x, -sign * w.y);
+        const float2 w = tw[i];
+        const int q = pinv[i];
+        tw_dst[i] = make_float2(w.x, -sign * w.y);
+        pinv_dst[i] = q;
     }
 }
 
@@ -324,8 +327,7 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
     // type-1 column passes, which read every row, were slower that way.
     const bool early = MASK == 0 && a.rows_in < a.L;
     if (early) load_chunk(threadIdx.x);
-    load_table(tw, a.tw, a.L, a.sign);
-    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
+    load_tables(tw, spinv, a.tw, a.pinv, a.L, a.sign);
     __syncthreads();
     // pruned input: the FFT positions no input row maps to, [rows_in - half_in,
     // L - half_in) with in_map and [rows_in, L) without, start as zeros; the stores
@@ -443,8 +445,7 @@ __device__ __forceinline__ void fft_rows_pad_body(const RowArgs &a, int64_t line
     const int ncl = (int)min((int64_t)a.CL, a.nc - c0);
     const int ls = a.L + 1;
     int *spinv = (int *)(buf + a.CL * ls);
-    load_table(tw, a.tw, a.L, a.sign);
-    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
+    load_tables(tw, spinv, a.tw, a.pinv, a.L, a.sign);
     for (int e = threadIdx.x; e < ncl * ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
     const int64_t n0 = (uint32_t)line / (uint32_t)a.m_b, n1 = line - n0 * a.m_b; // line < 2^31
@@ -534,8 +535,7 @@ __device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t lin
     const int64_t off = n0 * a.s0 + n1 * a.L;
     const float rncl = 1.0f / (float)ncl, rm = 1.0f / (float)a.m, rL = 1.0f / (float)a.L;
     int *spinv = (int *)(buf + a.CL * ls);
-    load_table(tw, a.tw, a.L, a.sign);
-    for (int i = threadIdx.x; i < a.L; i += blockDim.x) spinv[i] = a.pinv[i];
+    load_tables(tw, spinv, a.tw, a.pinv, a.L, a.sign);
     __syncthreads();
     const int gw = grid_pairs(a) ? 1 : 0; // log2 of the elements per lane
     const int ngp = (ncl * a.L) >> gw;
@@ -588,31 +588,52 @@ __device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t lin
         return;
     }
     const int iw = img_pairs(a, a.img_out, c0, ncl) ? 1 : 0;
-    auto get = [&](int n, int c) {
+    auto get = [&](int n, int c, float dl) {
         int p = n - half;
         if (p < 0) p += a.L;
-        const float d = d01 * a.deap_last[n];
+        const float d = d01 * dl;
         const float2 v = buf[c * ls + p];
         return make_float2(v.x * d, v.y * d);
     };
-    for (int q = threadIdx.x; q < (a.m * ncl) >> iw; q += blockDim.x) {
-        const int e = q << iw;
-        int n, c;
-        if (cl) {
-            n = fdiv(e, ncl, rncl);
-            c = e - n * ncl;
-        } else {
-            c = fdiv(e, a.m, rm);
-            n = e - c * a.m;
+    // the deapodization factors of kCropSteps output steps are loaded together
+    constexpr int kCropSteps = 2;
+    const int nq = (a.m * ncl) >> iw;
+    for (int q0 = threadIdx.x; q0 < nq; q0 += kCropSteps * blockDim.x) {
+        int nn[kCropSteps], cc[kCropSteps];
+        float dlo[kCropSteps], dhi[kCropSteps];
+#pragma unroll
+        for (int u = 0; u < kCropSteps; ++u) {
+            const int q = q0 + u * blockDim.x, e = q << iw;
+            int n, c;
+            if (cl) {
+                n = fdiv(e, ncl, rncl);
+                c = e - n * ncl;
+            } else {
+                c = fdiv(e, a.m, rm);
+                n = e - c * a.m;
+            }
+            nn[u] = n;
+            cc[u] = c;
+            dlo[u] = dhi[u] = 0.f;
+            if (q < nq) {
+                dlo[u] = a.deap_last[n];
+                if (iw && !cl) dhi[u] = a.deap_last[n + 1];
+            }
         }
-        float2 *dst = cl ? a.img_out + (line * a.m + n) * a.nc + c0 + c
-                         : a.img_out + (c0 + c) * a.nlines * a.m + line * a.m + n;
-        const float2 lo = get(n, c);
-        if (iw) {
-            const float2 hi = get(n + (cl ? 0 : 1), c + (cl ? 1 : 0));
-            store_stream(dst, make_float4(lo.x, lo.y, hi.x, hi.y));
-        } else {
-            *dst = lo;
+#pragma unroll
+        for (int u = 0; u < kCropSteps; ++u) {
+            const int q = q0 + u * blockDim.x;
+            if (q >= nq) continue;
+            const int n = nn[u], c = cc[u];
+            float2 *dst = cl ? a.img_out + (line * a.m + n) * a.nc + c0 + c
+                             : a.img_out + (c0 + c) * a.nlines * a.m + line * a.m + n;
+            const float2 lo = get(n, c, dlo[u]);
+            if (iw) {
+                const float2 hi = cl ? get(n, c + 1, dlo[u]) : get(n + 1, c, dhi[u]);
+                store_stream(dst, make_float4(lo.x, lo.y, hi.x, hi.y));
+            } else {
+                *dst = lo;
+            }
         }
     }
 }

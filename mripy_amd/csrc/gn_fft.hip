@@ -261,14 +261,16 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
     const int cbw = CB >> ws, cbws = cbs - ws;
     const int total_in = a.rows_in * cbw;
     // Row masks of this lane's columns: the lane index fixes the column (blockDim.x is a
-    // multiple of the columns per row), so one word per column serves every row.  They
-    // are fetched before the table staging so that both loads overlap.
+    // multiple of the columns per row), so one word per column serves every row.  Both
+    // words are fetched (for in-range columns) before the table staging and selected
+    // after it, so that the mask and table loads are in flight together.
     const float rmrow = 1.0f / (float)a.mrow;
-    unsigned long long mk0 = ~0ull, mk1 = ~0ull;
+    const int lgm = (a.mrow & (a.mrow - 1)) == 0 ? __builtin_ctz(a.mrow) : -1; // row tiles of 2^lgm rows
+    unsigned long long mk0 = ~0ull, mk1 = ~0ull, mw0 = 0ull, mw1 = 0ull;
     if (MASK) {
         const int j = (threadIdx.x & (cbw - 1)) << ws;
-        if (j < ncb) mk0 = col_mask(a, a.rmask, col0 + j);
-        mk1 = (v2 && j + 1 < ncb) ? col_mask(a, a.rmask, col0 + j + 1) : mk0;
+        mw0 = col_mask(a, a.rmask, col0 + min(j, ncb - 1));
+        mw1 = col_mask(a, a.rmask, col0 + min(j + 1, ncb - 1));
     }
     int *spinv = (int *)(buf + a.L * estride); // digit-reversal map, staged in LDS
     // kColLoadUnroll independent global loads in flight per thread before the LDS stores.
@@ -293,7 +295,7 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
                 if (e < total_in && j < ncb) {
                     uint32_t on = 3u;
                     if (MASK == 1) { // cells of tiles the spread did not write enter as zeros
-                        const int b = fdiv(n, a.mrow, rmrow);
+                        const int b = lgm >= 0 ? n >> lgm : fdiv(n, a.mrow, rmrow);
                         on = (uint32_t)((mk0 >> b) & 1ull) | ((uint32_t)((mk1 >> b) & 1ull) << 1);
                         onb |= on << (2 * u);
                     }
@@ -309,7 +311,7 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
                 if (e < total_in && j < ncb) {
                     uint32_t on = 1u;
                     if (MASK == 1) {
-                        const int b = fdiv(n, a.mrow, rmrow);
+                        const int b = lgm >= 0 ? n >> lgm : fdiv(n, a.mrow, rmrow);
                         on = (uint32_t)((mk0 >> b) & 1ull);
                         onb |= on << (2 * u);
                     }
@@ -329,6 +331,11 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
     if (early) load_chunk(threadIdx.x);
     load_tables(tw, spinv, a.tw, a.pinv, a.L, a.sign);
     __syncthreads();
+    if (MASK) {
+        const int j = (threadIdx.x & (cbw - 1)) << ws;
+        if (j < ncb) mk0 = mw0;
+        mk1 = (v2 && j + 1 < ncb) ? mw1 : mk0;
+    }
     // pruned input: the FFT positions no input row maps to, [rows_in - half_in,
     // L - half_in) with in_map and [rows_in, L) without, start as zeros; the stores
     // below write every other slot, so no barrier separates the two

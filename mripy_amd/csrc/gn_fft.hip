@@ -77,42 +77,30 @@ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x
 // multiply by (sign * i)
 __device__ inline float2 mul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
 
-// One radix-r butterfly on LDS elements at idx0 + k*step_q (k < r) of a line, after
-// twiddling element k by w_g^{t k}: tw[(t k L / g)] (the table in LDS already holds
-// the twiddles of this transform's sign).  TW = false for the first stage (g = r, all
-// twiddles 1).  Otherwise every element is multiplied, t = 0 included (tw[0] = 1 and
-// x * 1 - y * 0 = x exactly), so the lanes of a wave take no per-element branches.
-template <int R, bool TW>
-__device__ __forceinline__ void butterfly(float2 *buf, int base, int qstride, const float2 *tw, int twstep, int t, float sign)
+// Radix-R DFT of v[0..R) in registers (e^{sign 2 pi i / R} roots): v[k] becomes
+// output k.
+template <int R>
+__device__ __forceinline__ void dft_small(float2 *v, float sign)
 {
-    float2 v[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) v[k] = buf[base + k * qstride];
-    if (TW) {
-        float2 w[R];
-#pragma unroll
-        for (int k = 1; k < R; ++k) w[k] = tw[t * k * twstep];
-#pragma unroll
-        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], w[k]);
-    }
     if (R == 2) {
-        buf[base] = cadd(v[0], v[1]);
-        buf[base + qstride] = csub(v[0], v[1]);
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
     } else if (R == 4) {
         const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
         const float2 s13 = cadd(v[1], v[3]), d13 = mul_si(csub(v[1], v[3]), sign);
-        buf[base] = cadd(s02, s13);
-        buf[base + qstride] = cadd(d02, d13);
-        buf[base + 2 * qstride] = csub(s02, s13);
-        buf[base + 3 * qstride] = csub(d02, d13);
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, d13);
     } else if (R == 3) {
         const float c1 = -0.5f, s1 = 0.86602540378443864676f * sign;
         const float2 a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
-        buf[base] = cadd(v[0], a);
         const float2 m = make_float2(v[0].x + c1 * a.x, v[0].y + c1 * a.y);
         const float2 r = make_float2(-s1 * b.y, s1 * b.x);
-        buf[base + qstride] = cadd(m, r);
-        buf[base + 2 * qstride] = csub(m, r);
+        v[0] = cadd(v[0], a);
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
     } else if (R == 8) {
         // X[m] = E[m] + w^m O[m], X[m + 4] = E[m] - w^m O[m]: E, O the 4-point DFTs of
         // the even and odd inputs, w = e^{sign 2 pi i / 8}
@@ -127,29 +115,59 @@ __device__ __forceinline__ void butterfly(float2 *buf, int base, int qstride, co
         const float2 t1 = make_float2(r2 * (o1.x - sign * o1.y), r2 * (o1.y + sign * o1.x));
         const float2 t2 = mul_si(o2, sign);
         const float2 t3 = make_float2(r2 * (-o3.x - sign * o3.y), r2 * (-o3.y + sign * o3.x));
-        buf[base] = cadd(e0, o0);
-        buf[base + 4 * qstride] = csub(e0, o0);
-        buf[base + qstride] = cadd(e1, t1);
-        buf[base + 5 * qstride] = csub(e1, t1);
-        buf[base + 2 * qstride] = cadd(e2, t2);
-        buf[base + 6 * qstride] = csub(e2, t2);
-        buf[base + 3 * qstride] = cadd(e3, t3);
-        buf[base + 7 * qstride] = csub(e3, t3);
+        v[0] = cadd(e0, o0);
+        v[4] = csub(e0, o0);
+        v[1] = cadd(e1, t1);
+        v[5] = csub(e1, t1);
+        v[2] = cadd(e2, t2);
+        v[6] = csub(e2, t2);
+        v[3] = cadd(e3, t3);
+        v[7] = csub(e3, t3);
     } else if (R == 5) {
         const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
         const float s1 = 0.95105651629515357212f * sign, s2 = 0.58778525229247312917f * sign;
         const float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
         const float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
-        buf[base] = cadd(v[0], cadd(a1, a2));
         const float2 m1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
         const float2 m2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
         // r1 = i (s1 b1 + s2 b2), r2 = i (s2 b1 - s1 b2)
         const float2 r1 = make_float2(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
         const float2 r2 = make_float2(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
-        buf[base + qstride] = cadd(m1, r1);
-        buf[base + 4 * qstride] = csub(m1, r1);
-        buf[base + 2 * qstride] = cadd(m2, r2);
-        buf[base + 3 * qstride] = csub(m2, r2);
+        v[0] = cadd(v[0], cadd(a1, a2));
+        v[1] = cadd(m1, r1);
+        v[4] = csub(m1, r1);
+        v[2] = cadd(m2, r2);
+        v[3] = csub(m2, r2);
+    }
+}
+
+// One radix-r butterfly on LDS elements at base + k*qstride (k < r) of a line, after
+// twiddling element k by w_g^{t k}: tw[(t k L / g)] (the table in LDS already holds
+// the twiddles of this transform's sign).  TW = false for the first stage (g = r, all
+// twiddles 1).  Otherwise every element is multiplied, t = 0 included (tw[0] = 1 and
+// x * 1 - y * 0 = x exactly), so the lanes of a wave take no per-element branches.
+// NL = 2: the same butterfly on two adjacent lines (elements base + k*qstride and the
+// one after it), which share the index arithmetic and the twiddle reads.
+template <int R, bool TW, int NL>
+__device__ __forceinline__ void butterfly(float2 *buf, int base, int qstride, const float2 *tw, int twstep, int t, float sign)
+{
+    float2 w[R];
+    if (TW) {
+#pragma unroll
+        for (int k = 1; k < R; ++k) w[k] = tw[t * k * twstep];
+    }
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        float2 v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = buf[base + k * qstride + l];
+        if (TW) {
+#pragma unroll
+            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], w[k]);
+        }
+        dft_small<R>(v, sign);
+#pragma unroll
+        for (int k = 0; k < R; ++k) buf[base + k * qstride + l] = v[k];
     }
 }
 
@@ -164,44 +182,48 @@ __device__ inline int fdiv(int n, int d, float r)
 }
 
 // In-place FFT of `nlines` lines of length L held in LDS: element i of line j at
-// buf[j * lstride + i * estride].  All threads of the block take part.
+// buf[j * lstride + i * estride].  All threads of the block take part.  NL = 2 (lines
+// adjacent, lstride = 1, nlines even): a thread takes the same butterfly on lines j,
+// j + 1.
+template <int NL>
 __device__ __forceinline__ void fft_lines(float2 *buf, int nlines, int lstride, int estride, int L, const Stages &st,
-                                 const float2 *tw, float sign)
+                                          const float2 *tw, float sign)
 {
+    const int nw = nlines / NL; // line groups
     // fdiv corrects its quotient estimate, so approximate reciprocals suffice
-    const float rl = __builtin_amdgcn_rcpf((float)nlines);
+    const float rl = __builtin_amdgcn_rcpf((float)nw);
     // power-of-two line counts and butterfly groups index with shifts
-    const int lgl = (nlines & (nlines - 1)) == 0 ? __builtin_ctz(nlines) : -1;
+    const int lgl = (nw & (nw - 1)) == 0 ? __builtin_ctz(nw) : -1;
     for (int s = 0; s < st.n; ++s) {
         const int r = st.radix[s];
         const int q = st.q[s];         // butterfly span before this stage
         const int g = r * q;
         const int twstep = st.tws[s];  // L / g
-        const int nbl = st.nb[s] * nlines; // butterflies of all lines
+        const int nbl = st.nb[s] * nw; // butterflies of all line groups
         const float rq = __builtin_amdgcn_rcpf((float)q);
         const int lgq = (q & (q - 1)) == 0 ? __builtin_ctz(q) : -1;
         for (int w = threadIdx.x; w < nbl; w += blockDim.x) {
-            // lines fastest: neighbouring threads work on neighbouring lines
-            const int bidx = lgl >= 0 ? (w >> lgl) : fdiv(w, nlines, rl);
-            const int j = w - bidx * nlines;
+            // line groups fastest: neighbouring threads work on neighbouring lines
+            const int bidx = lgl >= 0 ? (w >> lgl) : fdiv(w, nw, rl);
+            const int j = (w - bidx * nw) * NL;
             const int blk = lgq >= 0 ? (bidx >> lgq) : fdiv(bidx, q, rq);
             const int t = bidx - blk * q;
             const int base = j * lstride + (blk * g + t) * estride;
             if (s == 0) {
                 switch (r) {
-                case 2: butterfly<2, false>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 3: butterfly<3, false>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 4: butterfly<4, false>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 8: butterfly<8, false>(buf, base, q * estride, tw, twstep, t, sign); break;
-                default: butterfly<5, false>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 2: butterfly<2, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 3: butterfly<3, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 4: butterfly<4, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 8: butterfly<8, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                default: butterfly<5, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
                 }
             } else {
                 switch (r) {
-                case 2: butterfly<2, true>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 3: butterfly<3, true>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 4: butterfly<4, true>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 8: butterfly<8, true>(buf, base, q * estride, tw, twstep, t, sign); break;
-                default: butterfly<5, true>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 2: butterfly<2, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 3: butterfly<3, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 4: butterfly<4, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 8: butterfly<8, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                default: butterfly<5, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
                 }
             }
         }
@@ -373,7 +395,11 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
         }
     }
     __syncthreads();
-    fft_lines(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
+    // adjacent columns in pairs: one index computation and one twiddle read serve both
+    if ((CB & 1) == 0)
+        fft_lines<2>(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
+    else
+        fft_lines<1>(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
     const int half_out = a.rows_out / 2;
     for (int e = threadIdx.x; e < a.rows_out * cbw; e += blockDim.x) {
         const int n = e >> cbws, j = (e & (cbw - 1)) << ws;
@@ -522,7 +548,7 @@ __device__ __forceinline__ void fft_rows_pad_body(const RowArgs &a, int64_t line
         }
     }
     __syncthreads();
-    fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
+    fft_lines<1>(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
     const int64_t off = n0 * a.s0 + n1 * a.L;
     for (int e = threadIdx.x; e < ncl * a.L; e += blockDim.x) {
         const int c = fdiv(e, a.L, rL), k = e - c * a.L;
@@ -573,7 +599,7 @@ __device__ __forceinline__ void fft_rows_crop_body(const RowArgs &a, int64_t lin
         }
     }
     __syncthreads();
-    fft_lines(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
+    fft_lines<1>(buf, ncl, ls, 1, a.L, a.st, tw, a.sign);
     const float d01 = a.scale * a.deap_a[n0] * (a.deap_b ? a.deap_b[n1] : 1.0f);
     const int half = a.m / 2;
     const bool cl = a.layout == GN_COIL_LAST;

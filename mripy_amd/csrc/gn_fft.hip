@@ -146,10 +146,11 @@ __device__ __forceinline__ void dft_small(float2 *v, float sign)
 // the twiddles of this transform's sign).  TW = false for the first stage (g = r, all
 // twiddles 1).  Otherwise every element is multiplied, t = 0 included (tw[0] = 1 and
 // x * 1 - y * 0 = x exactly), so the lanes of a wave take no per-element branches.
-// NL = 2: the same butterfly on two adjacent lines (elements base + k*qstride and the
-// one after it), which share the index arithmetic and the twiddle reads.
+// NL = 2: the same butterfly on two lines (elements base + k*qstride and lstep after
+// them), which share the index arithmetic and the twiddle reads.
 template <int R, bool TW, int NL>
-__device__ __forceinline__ void butterfly(float2 *buf, int base, int qstride, const float2 *tw, int twstep, int t, float sign)
+__device__ __forceinline__ void butterfly(float2 *buf, int base, int qstride, int lstep, const float2 *tw, int twstep,
+                                          int t, float sign)
 {
     float2 w[R];
     if (TW) {
@@ -160,14 +161,14 @@ __device__ __forceinline__ void butterfly(float2 *buf, int base, int qstride, co
     for (int l = 0; l < NL; ++l) {
         float2 v[R];
 #pragma unroll
-        for (int k = 0; k < R; ++k) v[k] = buf[base + k * qstride + l];
+        for (int k = 0; k < R; ++k) v[k] = buf[base + k * qstride + l * lstep];
         if (TW) {
 #pragma unroll
             for (int k = 1; k < R; ++k) v[k] = cmul(v[k], w[k]);
         }
         dft_small<R>(v, sign);
 #pragma unroll
-        for (int k = 0; k < R; ++k) buf[base + k * qstride + l] = v[k];
+        for (int k = 0; k < R; ++k) buf[base + k * qstride + l * lstep] = v[k];
     }
 }
 
@@ -182,9 +183,9 @@ __device__ inline int fdiv(int n, int d, float r)
 }
 
 // In-place FFT of `nlines` lines of length L held in LDS: element i of line j at
-// buf[j * lstride + i * estride].  All threads of the block take part.  NL = 2 (lines
-// adjacent, lstride = 1, nlines even): a thread takes the same butterfly on lines j,
-// j + 1.
+// buf[j * lstride + i * estride].  All threads of the block take part.  NL = 2 (nlines
+// even): a thread takes the same butterfly on lines j and j + nlines / 2, so that the
+// lanes of a wave still read consecutive lines in each of the two reads.
 template <int NL>
 __device__ __forceinline__ void fft_lines(float2 *buf, int nlines, int lstride, int estride, int L, const Stages &st,
                                           const float2 *tw, float sign)
@@ -205,25 +206,25 @@ __device__ __forceinline__ void fft_lines(float2 *buf, int nlines, int lstride, 
         for (int w = threadIdx.x; w < nbl; w += blockDim.x) {
             // line groups fastest: neighbouring threads work on neighbouring lines
             const int bidx = lgl >= 0 ? (w >> lgl) : fdiv(w, nw, rl);
-            const int j = (w - bidx * nw) * NL;
+            const int j = w - bidx * nw;
             const int blk = lgq >= 0 ? (bidx >> lgq) : fdiv(bidx, q, rq);
             const int t = bidx - blk * q;
             const int base = j * lstride + (blk * g + t) * estride;
             if (s == 0) {
                 switch (r) {
-                case 2: butterfly<2, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 3: butterfly<3, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 4: butterfly<4, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 8: butterfly<8, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                default: butterfly<5, false, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 2: butterfly<2, false, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                case 3: butterfly<3, false, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                case 4: butterfly<4, false, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                case 8: butterfly<8, false, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                default: butterfly<5, false, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
                 }
             } else {
                 switch (r) {
-                case 2: butterfly<2, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 3: butterfly<3, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 4: butterfly<4, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                case 8: butterfly<8, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
-                default: butterfly<5, true, NL>(buf, base, q * estride, tw, twstep, t, sign); break;
+                case 2: butterfly<2, true, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                case 3: butterfly<3, true, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                case 4: butterfly<4, true, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                case 8: butterfly<8, true, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
+                default: butterfly<5, true, NL>(buf, base, q * estride, nw * lstride, tw, twstep, t, sign); break;
                 }
             }
         }
@@ -395,7 +396,7 @@ __device__ __forceinline__ void fft_cols_body(const ColArgs &a, int64_t colblk, 
         }
     }
     __syncthreads();
-    // adjacent columns in pairs: one index computation and one twiddle read serve both
+    // columns in pairs: one index computation and one set of twiddle reads serve both
     if ((CB & 1) == 0)
         fft_lines<2>(buf, CB, lstride, estride, a.L, a.st, tw, a.sign);
     else
